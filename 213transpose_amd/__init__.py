@@ -1,0 +1,198 @@
+"""MI355X (gfx950) in-place transpose engine — Python host mirror.
+
+The compute path is libinplace_hip.so (hand-written HIP kernels behind a C ABI,
+declared in include/inplace_transpose.h).  This module binds that ABI with
+ctypes and mirrors the reference's interface (include/transpose.h):
+
+    transpose(data, d1, d2, d3)        inplace::transpose<T>      (src/transpose.cu:99-124)
+    c2r.transpose(data, d1, d2, d3)    inplace::c2r::transpose<T> (src/transpose.cu:30-47)
+    r2c.transpose(data, d1, d2, d3)    inplace::r2c::transpose<T> (src/transpose.cu:70-89)
+    openmp_style(row_major, data, m, n) inplace::openmp::transpose (include/openmp.h:7-8)
+
+`data` is a contiguous CUDA (HIP) tensor of 4- or 8-byte elements holding d3
+slices of d2 rows x d1 columns; every slice is replaced by its transpose.  Work
+is enqueued on torch's current stream.  There is no CPU fallback: if the
+library is missing or a call fails, a TransposeError is raised.
+
+The package directory name starts with a digit, so import it with
+importlib.import_module("213transpose_amd").
+"""
+import ctypes
+import os
+import types
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libinplace_hip.so")
+
+VARIANT_AUTO, VARIANT_SLICE_LDS, VARIANT_SQUARE_SWAP, VARIANT_C2R, VARIANT_R2C = range(5)
+VARIANT_NAMES = {VARIANT_AUTO: "auto", VARIANT_SLICE_LDS: "slice_lds",
+                 VARIANT_SQUARE_SWAP: "square_swap", VARIANT_C2R: "c2r", VARIANT_R2C: "r2c"}
+N_VARIANTS = 5
+
+# Map ids of ipt_debug_eval_map.
+MAP_C2R_PRE, MAP_C2R_POST, MAP_R2C_PRE, MAP_R2C_POST, MAP_C2R_SHUFFLE, MAP_R2C_SHUFFLE = range(6)
+
+STATUS = {0: "IPT_OK", 1: "IPT_ERR_INVALID_ARG", 2: "IPT_ERR_UNSUPPORTED", 3: "IPT_ERR_TOO_LARGE",
+          4: "IPT_ERR_HIP", 5: "IPT_ERR_NO_DEVICE", 6: "IPT_ERR_OUT_OF_MEMORY", 7: "IPT_ERR_IO",
+          8: "IPT_ERR_NOT_READY"}
+
+
+class TransposeError(RuntimeError):
+    def __init__(self, status, what, detail=""):
+        self.status = status
+        msg = f"{what}: {STATUS.get(status, status)}"
+        if detail:
+            msg += f" ({detail})"
+        super().__init__(msg)
+
+
+class Tune(ctypes.Structure):
+    """ipt_tune: zero fields mean library default."""
+    _fields_ = [("lds_bytes", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("strip", ctypes.c_int32), ("grid_mult", ctypes.c_int32)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("R", "C", "c", "a", "b", "k", "q")] + \
+               [("heuristic_variant", ctypes.c_int32), ("index_bits", ctypes.c_int32)]
+
+
+_i64, _vp, _int = ctypes.c_int64, ctypes.c_void_p, ctypes.c_int
+_lib = None
+
+
+def lib():
+    """Load libinplace_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TransposeError(-1, "libinplace_hip.so not built",
+                             f"expected {LIB_PATH}; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    L.ipt_status_string.restype = ctypes.c_char_p
+    L.ipt_status_string.argtypes = [_int]
+    L.ipt_last_error.restype = ctypes.c_char_p
+    L.ipt_last_error.argtypes = []
+    for name in ("ipt_transpose_213", "ipt_c2r_transpose", "ipt_r2c_transpose"):
+        f = getattr(L, name)
+        f.restype = _int
+        f.argtypes = [_vp, _int, _i64, _i64, _i64, _vp]
+    L.ipt_transpose.restype = _int
+    L.ipt_transpose.argtypes = [_int, _vp, _int, _i64, _i64, _vp]
+    L.ipt_transpose_variant.restype = _int
+    L.ipt_transpose_variant.argtypes = [_vp, _int, _i64, _i64, _i64, _int,
+                                        ctypes.POINTER(Tune), _vp]
+    L.ipt_variant_feasible.restype = _int
+    L.ipt_variant_feasible.argtypes = [_int, _int, _i64, _i64, _i64]
+    L.ipt_plan_query.restype = _int
+    L.ipt_plan_query.argtypes = [_int, _i64, _i64, _i64, ctypes.POINTER(PlanInfo)]
+    L.ipt_debug_eval_map.restype = _int
+    L.ipt_debug_eval_map.argtypes = [_int, _i64, _i64, _int, _vp, _vp, _i64, _vp]
+    L.ipt_debug_run_pass.restype = _int
+    L.ipt_debug_run_pass.argtypes = [_vp, _int, _i64, _i64, _i64, _int, ctypes.POINTER(Tune), _vp]
+    L.ipt_version.restype = ctypes.c_char_p
+    L.ipt_version.argtypes = []
+    _lib = L
+    return L
+
+
+def _check(status, what):
+    if status != 0:
+        detail = lib().ipt_last_error().decode(errors="replace")
+        raise TransposeError(status, what, detail)
+
+
+def _stream_ptr(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _args(data, d1, d2, d3):
+    if not data.is_cuda:
+        raise TransposeError(1, "data must be a GPU tensor (no CPU path)")
+    if not data.is_contiguous():
+        raise TransposeError(1, "data must be contiguous")
+    esz = data.element_size()
+    if esz not in (4, 8):
+        raise TransposeError(1, f"element size {esz} not supported (4 or 8 bytes)")
+    if d1 * d2 * d3 != data.numel():
+        raise TransposeError(1, f"d1*d2*d3 = {d1 * d2 * d3} != numel {data.numel()}")
+    return ctypes.c_void_p(data.data_ptr()), esz
+
+
+def transpose(data, d1, d2, d3=1, stream=None):
+    """inplace::transpose<T>: transpose each of d3 slices (d2 rows x d1 cols) in place."""
+    ptr, esz = _args(data, d1, d2, d3)
+    _check(lib().ipt_transpose_213(ptr, esz, d1, d2, d3, _stream_ptr(stream)), "ipt_transpose_213")
+    return data
+
+
+def _c2r(data, d1, d2, d3=1, stream=None):
+    ptr, esz = _args(data, d1, d2, d3)
+    _check(lib().ipt_c2r_transpose(ptr, esz, d1, d2, d3, _stream_ptr(stream)), "ipt_c2r_transpose")
+    return data
+
+
+def _r2c(data, d1, d2, d3=1, stream=None):
+    ptr, esz = _args(data, d1, d2, d3)
+    _check(lib().ipt_r2c_transpose(ptr, esz, d1, d2, d3, _stream_ptr(stream)), "ipt_r2c_transpose")
+    return data
+
+
+# inplace::c2r::transpose: slices of d2 rows x d1 cols.
+c2r = types.SimpleNamespace(transpose=_c2r)
+# inplace::r2c::transpose: slices of d1 rows x d2 cols (reference argument order).
+r2c = types.SimpleNamespace(transpose=_r2c)
+
+
+def openmp_style(row_major, data, m, n, stream=None):
+    """inplace::openmp::transpose(row_major, data, m, n, tmp) on the GPU."""
+    ptr, esz = _args(data, m, n, 1)
+    _check(lib().ipt_transpose(int(bool(row_major)), ptr, esz, m, n, _stream_ptr(stream)),
+           "ipt_transpose")
+    return data
+
+
+def transpose_variant(data, d1, d2, d3, variant, tune=None, stream=None):
+    """Force one variant (and optional Tune knobs)."""
+    ptr, esz = _args(data, d1, d2, d3)
+    t = ctypes.byref(tune) if tune is not None else None
+    _check(lib().ipt_transpose_variant(ptr, esz, d1, d2, d3, variant, t, _stream_ptr(stream)),
+           f"ipt_transpose_variant({VARIANT_NAMES.get(variant, variant)})")
+    return data
+
+
+def feasible(variant, esz, d1, d2, d3=1):
+    return bool(lib().ipt_variant_feasible(variant, esz, d1, d2, d3))
+
+
+def plan_query(esz, d1, d2, d3=1):
+    info = PlanInfo()
+    _check(lib().ipt_plan_query(esz, d1, d2, d3, ctypes.byref(info)), "ipt_plan_query")
+    return {f: getattr(info, f) for f, _ in PlanInfo._fields_}
+
+
+def eval_map(map_id, R, C, index_bits, i, j):
+    """Host evaluation of the library's index map (numpy int64 arrays in/out)."""
+    import numpy as np
+    i = np.ascontiguousarray(i, dtype=np.int64)
+    j = np.ascontiguousarray(j, dtype=np.int64)
+    out = np.empty_like(i)
+    _check(lib().ipt_debug_eval_map(map_id, R, C, index_bits, i.ctypes.data, j.ctypes.data,
+                                    i.size, out.ctypes.data), "ipt_debug_eval_map")
+    return out
+
+
+def run_pass(data, R, C, d3, map_id, tune=None, stream=None):
+    """One gather pass on d3 slices viewed as R x C (per-pass parity / timing)."""
+    ptr, esz = _args(data, C, R, d3)
+    t = ctypes.byref(tune) if tune is not None else None
+    _check(lib().ipt_debug_run_pass(ptr, esz, R, C, d3, map_id, t, _stream_ptr(stream)),
+           f"ipt_debug_run_pass({map_id})")
+    return data
+
+
+def version():
+    return lib().ipt_version().decode()

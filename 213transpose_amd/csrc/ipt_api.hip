@@ -1,0 +1,257 @@
+// ipt_api.hip — C ABI, per-device context, dispatcher and launchers.
+//
+// The dispatcher mirrors inplace::transpose<T> (src/transpose.cu:99-124):
+// pick a decomposition for the slice shape, then run its passes as
+// stream-ordered kernels.  Unlike the reference it never prefetches managed
+// memory, never prints, never allocates per call (scratch is cached per
+// device) and reports every failure as an ipt_status.
+#include "../../include/inplace_transpose.h"
+#include "ipt_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+
+namespace ipt {
+
+// ---------------------------------------------------------------- errors
+
+static thread_local char g_last_error[256] = "";
+
+void set_error(const char* what, hipError_t e) {
+    snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- devices
+
+static std::mutex g_ctx_mu;
+static std::map<int, std::unique_ptr<DeviceCtx>> g_ctx;
+
+ipt_status get_ctx(DeviceCtx** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("hipGetDevice", e);
+        return IPT_ERR_NO_DEVICE;
+    }
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    auto it = g_ctx.find(dev);
+    if (it != g_ctx.end()) {
+        *out = it->second.get();
+        return IPT_OK;
+    }
+    std::unique_ptr<DeviceCtx> c(new DeviceCtx());
+    c->dev = dev;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) {
+        set_error("hipGetDeviceProperties", e);
+        return IPT_ERR_NO_DEVICE;
+    }
+    c->ncu = prop.multiProcessorCount;
+    c->lds_max = (int64_t)prop.sharedMemPerBlock;
+    snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+    *out = c.get();
+    g_ctx[dev] = std::move(c);
+    return IPT_OK;
+}
+
+// Scratch is owned by the device context and shared by every call on that
+// device.  A user on another stream first waits for the previous user's last
+// kernel (event), so reuse is ordered; growth synchronises the device once.
+ipt_status scratch_acquire(DeviceCtx* ctx, size_t bytes, hipStream_t s, void** ptr) {
+    if (ctx->scratch_bytes < bytes) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) { set_error("hipDeviceSynchronize", e); return IPT_ERR_HIP; }
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        size_t want = std::max(bytes, (size_t)64 << 20);
+        e = hipMalloc(&ctx->scratch, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            e = hipMalloc(&ctx->scratch, bytes);
+            want = bytes;
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->scratch = nullptr;
+            set_error("hipMalloc(scratch)", e);
+            return IPT_ERR_OUT_OF_MEMORY;
+        }
+        ctx->scratch_bytes = want;
+        ctx->scratch_stream = s;
+    }
+    if (ctx->scratch_event_valid && ctx->scratch_stream != s) {
+        hipError_t e = hipStreamWaitEvent(s, ctx->scratch_event, 0);
+        if (e != hipSuccess) { set_error("hipStreamWaitEvent", e); return IPT_ERR_HIP; }
+    }
+    *ptr = ctx->scratch;
+    return IPT_OK;
+}
+
+ipt_status scratch_release(DeviceCtx* ctx, hipStream_t s) {
+    if (!ctx->scratch_event) {
+        hipError_t e = hipEventCreateWithFlags(&ctx->scratch_event, hipEventDisableTiming);
+        if (e != hipSuccess) { set_error("hipEventCreate", e); return IPT_ERR_HIP; }
+    }
+    hipError_t e = hipEventRecord(ctx->scratch_event, s);
+    if (e != hipSuccess) { set_error("hipEventRecord", e); return IPT_ERR_HIP; }
+    ctx->scratch_event_valid = true;
+    ctx->scratch_stream = s;
+    return IPT_OK;
+}
+
+// ---------------------------------------------------------------- shapes
+
+Shape make_shape(int esz, int64_t d1, int64_t d2, int64_t d3) {
+    Shape s;
+    s.esz = esz;
+    s.d1 = d1;
+    s.d2 = d2;
+    s.d3 = d3;
+    return s;
+}
+
+ipt_status validate(const void* data, int esz, int64_t d1, int64_t d2, int64_t d3) {
+    if (!data) return IPT_ERR_INVALID_ARG;
+    if (esz != 4 && esz != 8) return IPT_ERR_INVALID_ARG;
+    if (d1 <= 0 || d2 <= 0 || d3 <= 0) return IPT_ERR_INVALID_ARG;
+    // Total bytes must fit comfortably in 63 bits, and one slice in 2^40
+    // elements (the 64-bit plan keeps every numerator below 2^63).
+    unsigned __int128 slice = (unsigned __int128)d1 * (unsigned __int128)d2;
+    unsigned __int128 total = slice * (unsigned __int128)d3 * (unsigned)esz;
+    if (slice > ((unsigned __int128)1 << 40) || total > ((unsigned __int128)1 << 62))
+        return IPT_ERR_TOO_LARGE;
+    if ((uintptr_t)data % (uintptr_t)esz) return IPT_ERR_INVALID_ARG;
+    return IPT_OK;
+}
+
+}  // namespace ipt
+
+using namespace ipt;
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" {
+
+const char* ipt_status_string(ipt_status s) {
+    switch (s) {
+    case IPT_OK: return "ok";
+    case IPT_ERR_INVALID_ARG: return "invalid argument";
+    case IPT_ERR_UNSUPPORTED: return "variant unsupported for this shape";
+    case IPT_ERR_TOO_LARGE: return "size beyond the index plan";
+    case IPT_ERR_HIP: return "HIP runtime error";
+    case IPT_ERR_NO_DEVICE: return "no usable GPU";
+    case IPT_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case IPT_ERR_IO: return "file missing or corrupt";
+    case IPT_ERR_NOT_READY: return "object not ready";
+    }
+    return "unknown status";
+}
+
+const char* ipt_last_error(void) { return g_last_error; }
+
+ipt_status ipt_transpose_variant(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
+                                 int variant, const ipt_tune* tune, void* stream) {
+    ipt_status st = validate(data, elem_bytes, d1, d2, d3);
+    if (st != IPT_OK) return st;
+    if (variant < 0 || variant >= IPT_VARIANT_COUNT) return IPT_ERR_INVALID_ARG;
+    DeviceCtx* ctx;
+    if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    Shape s = make_shape(elem_bytes, d1, d2, d3);
+    Tune t = resolve_tune(ctx, tune);
+    if (variant == IPT_VARIANT_AUTO) variant = choose_variant(ctx, s);
+    return run_variant(ctx, data, s, variant, t, (hipStream_t)stream);
+}
+
+ipt_status ipt_transpose_213(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
+                             void* stream) {
+    return ipt_transpose_variant(data, elem_bytes, d1, d2, d3, IPT_VARIANT_AUTO, nullptr, stream);
+}
+
+ipt_status ipt_c2r_transpose(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
+                             void* stream) {
+    return ipt_transpose_variant(data, elem_bytes, d1, d2, d3, IPT_VARIANT_C2R, nullptr, stream);
+}
+
+ipt_status ipt_r2c_transpose(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
+                             void* stream) {
+    // Reference argument order: slices of d1 rows x d2 columns, i.e. the 213
+    // transpose of (d1' = d2, d2' = d1) done with the R2C view R = d2, C = d1.
+    return ipt_transpose_variant(data, elem_bytes, d2, d1, d3, IPT_VARIANT_R2C, nullptr, stream);
+}
+
+ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m, int64_t n,
+                         void* stream) {
+    if (row_major) return ipt_transpose_213(data, elem_bytes, n, m, 1, stream);
+    return ipt_transpose_213(data, elem_bytes, m, n, 1, stream);
+}
+
+int ipt_variant_feasible(int variant, int elem_bytes, int64_t d1, int64_t d2, int64_t d3) {
+    if (elem_bytes != 4 && elem_bytes != 8) return 0;
+    if (d1 <= 0 || d2 <= 0 || d3 <= 0) return 0;
+    DeviceCtx* ctx;
+    if (get_ctx(&ctx) != IPT_OK) return 0;
+    return variant_feasible(ctx, make_shape(elem_bytes, d1, d2, d3), variant) ? 1 : 0;
+}
+
+ipt_status ipt_plan_query(int elem_bytes, int64_t d1, int64_t d2, int64_t d3, ipt_plan_info* out) {
+    if (!out) return IPT_ERR_INVALID_ARG;
+    if (elem_bytes != 4 && elem_bytes != 8) return IPT_ERR_INVALID_ARG;
+    if (d1 <= 0 || d2 <= 0 || d3 <= 0) return IPT_ERR_INVALID_ARG;
+    Shape s = make_shape(elem_bytes, d1, d2, d3);
+    // View of the heuristic's decomposition: C2R keeps (R, C) = (d2, d1),
+    // R2C uses (d1, d2) (src/transpose.cu:116-123).
+    const bool c2r = d1 >= d2;
+    HostPlan h = HostPlan::make(c2r ? d2 : d1, c2r ? d1 : d2);
+    out->R = h.R; out->C = h.C; out->c = h.c; out->a = h.a; out->b = h.b; out->k = h.k; out->q = h.q;
+    out->index_bits = h.fits_index32() ? 32 : 64;
+    DeviceCtx* ctx = nullptr;
+    out->heuristic_variant = (get_ctx(&ctx) == IPT_OK) ? choose_variant(ctx, s)
+                                                       : choose_variant(nullptr, s);
+    return IPT_OK;
+}
+
+ipt_status ipt_debug_eval_map(int map_id, int64_t R, int64_t C, int index_bits, const int64_t* i,
+                              const int64_t* j, int64_t n, int64_t* out) {
+    if (R <= 0 || C <= 0 || n < 0 || (n > 0 && (!i || !j || !out))) return IPT_ERR_INVALID_ARG;
+    if (map_id < 0 || map_id > MAP_R2C_SHUFFLE) return IPT_ERR_INVALID_ARG;
+    HostPlan h = HostPlan::make(R, C);
+    if (index_bits == 32) {
+        if (!h.fits_index32()) return IPT_ERR_UNSUPPORTED;
+        return eval_map_host<uint32_t>(map_id, h, i, j, n, out);
+    }
+    if (index_bits != 64) return IPT_ERR_INVALID_ARG;
+    return eval_map_host<uint64_t>(map_id, h, i, j, n, out);
+}
+
+ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, int64_t d3,
+                              int map_id, const ipt_tune* tune, void* stream) {
+    ipt_status st = validate(data, elem_bytes, C, R, d3);
+    if (st != IPT_OK) return st;
+    if (map_id < 0 || map_id > MAP_R2C_SHUFFLE) return IPT_ERR_INVALID_ARG;
+    DeviceCtx* ctx;
+    if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    return run_pass(ctx, data, elem_bytes, R, C, d3, map_id, resolve_tune(ctx, tune),
+                    (hipStream_t)stream);
+}
+
+const char* ipt_version(void) {
+    static char buf[128];
+    DeviceCtx* ctx;
+    if (get_ctx(&ctx) != IPT_OK) {
+        snprintf(buf, sizeof(buf), "inplace_hip (no device)");
+    } else {
+        snprintf(buf, sizeof(buf), "inplace_hip %s lds=%lld cus=%d", ctx->arch,
+                 (long long)ctx->lds_max, ctx->ncu);
+    }
+    return buf;
+}
+
+}  // extern "C"

@@ -1,0 +1,329 @@
+// ipt_dispatch.hip — variant choice and the launchers of every pass.
+//
+// Variants (ipt_variant):
+//   SLICE_LDS    whole slice staged in one workgroup's LDS, written back
+//                transposed: one read and one write of every element.
+//   SQUARE_SWAP  d1 == d2: tile pairs swapped through LDS, one sweep.
+//   C2R / R2C    the three-pass decomposition (src/transpose.cu:30-89), each
+//                pass a column or row gather; tiers per pass: LDS when the
+//                column strip / row fits, else scratch gather + copy back.
+#include "ipt_internal.h"
+#include "ipt_kernels.h"
+
+#include <algorithm>
+
+namespace ipt {
+
+#define IPT_LAUNCH_CHECK(what)                          \
+    do {                                                \
+        hipError_t e_ = hipGetLastError();              \
+        if (e_ != hipSuccess) {                         \
+            set_error(what, e_);                        \
+            return IPT_ERR_HIP;                         \
+        }                                               \
+    } while (0)
+
+#define IPT_HIP_CHECK(call)                             \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) {                         \
+            set_error(#call, e_);                       \
+            return IPT_ERR_HIP;                         \
+        }                                               \
+    } while (0)
+
+static int64_t lds_max_of(const DeviceCtx* ctx) { return ctx ? ctx->lds_max : kDefaultLds; }
+static int ncu_of(const DeviceCtx* ctx) { return ctx ? ctx->ncu : 256; }
+
+Tune resolve_tune(const DeviceCtx* ctx, const ipt_tune* t) {
+    Tune r;
+    r.lds_cap = (int)lds_max_of(ctx);
+    r.lds_bytes = (int)std::min<int64_t>(64 * 1024, lds_max_of(ctx));
+    r.block = 256;
+    r.strip = 64;
+    r.grid_mult = 2;
+    if (t) {
+        if (t->lds_bytes > 0) {
+            r.lds_cap = (int)std::min<int64_t>(t->lds_bytes, lds_max_of(ctx));
+            r.lds_bytes = std::min(r.lds_bytes, r.lds_cap);
+        }
+        if (t->block >= 64 && t->block <= 1024 && (t->block & (t->block - 1)) == 0) r.block = t->block;
+        if (t->strip > 0) r.strip = t->strip;
+        if (t->grid_mult > 0) r.grid_mult = t->grid_mult;
+    }
+    return r;
+}
+
+// Odd LDS row pitch for the whole-slice kernel.
+static int64_t slice_pitch(int64_t C) { return C | 1; }
+
+static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
+    return s.d2 * slice_pitch(s.d1) * s.esz <= lds_max_of(ctx);
+}
+
+bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant) {
+    switch (variant) {
+    case IPT_VARIANT_AUTO: return true;
+    case IPT_VARIANT_SLICE_LDS: return slice_lds_fits(ctx, s);
+    case IPT_VARIANT_SQUARE_SWAP: return s.d1 == s.d2;
+    case IPT_VARIANT_C2R:
+    case IPT_VARIANT_R2C: return true;
+    default: return false;
+    }
+}
+
+// Heuristic dispatch (the fallback when no learned selector is loaded).
+// Mirrors src/transpose.cu:105-123 for the decomposition choice (C2R when
+// d1 >= d2, else R2C with swapped arguments) and adds the one-sweep variants
+// first where they apply.
+int choose_variant(const DeviceCtx* ctx, const Shape& s) {
+    if (slice_lds_fits(ctx, s)) return IPT_VARIANT_SLICE_LDS;
+    if (s.d1 == s.d2) return IPT_VARIANT_SQUARE_SWAP;
+    return s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
+}
+
+// Workgroups for a grid-stride kernel: occupancy x CUs x grid_mult, capped by
+// the number of work items.
+template <typename K>
+static ipt_status grid_for(const DeviceCtx* ctx, K kernel, int block, size_t lds, uint64_t items,
+                           int grid_mult, int* grid) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds);
+    if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+    uint64_t g = (uint64_t)per_cu * (uint64_t)ncu_of(ctx) * (uint64_t)std::max(grid_mult, 1);
+    g = std::min<uint64_t>(g, items);
+    *grid = (int)std::max<uint64_t>(g, 1);
+    return IPT_OK;
+}
+
+template <typename K>
+static ipt_status allow_lds(K kernel, size_t lds) {
+    if (lds > 64 * 1024) {
+        IPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    return IPT_OK;
+}
+
+// ---------------------------------------------------------------- SLICE_LDS
+
+template <typename T>
+static ipt_status run_slice_lds(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t,
+                                hipStream_t st) {
+    const uint32_t R = (uint32_t)s.d2, C = (uint32_t)s.d1;
+    const uint32_t P = (uint32_t)slice_pitch(C);
+    const int64_t slice_bytes = (int64_t)R * P * sizeof(T);
+    int64_t budget = std::max<int64_t>(t.lds_bytes, slice_bytes);
+    uint32_t per = (uint32_t)std::max<int64_t>(1, budget / slice_bytes);
+    per = (uint32_t)std::min<uint64_t>(per, (uint64_t)s.d3);
+    const size_t lds = (size_t)per * slice_bytes;
+    auto k = k_slice_lds<T>;
+    ipt_status stt = allow_lds(k, lds);
+    if (stt != IPT_OK) return stt;
+    const uint64_t items = ((uint64_t)s.d3 + per - 1) / per;
+    int grid;
+    grid_for(ctx, k, t.block, lds, items, t.grid_mult, &grid);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(t.block), lds, st, data, R, C, P, (uint64_t)s.d3, per,
+                       Div32::make(R * C), Div32::make(C), Div32::make(R));
+    IPT_LAUNCH_CHECK("k_slice_lds");
+    return IPT_OK;
+}
+
+// ---------------------------------------------------------------- SQUARE_SWAP
+
+template <typename T>
+static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t,
+                             hipStream_t st) {
+    constexpr int TS = sizeof(T) == 4 ? 64 : 32;
+    const uint32_t N = (uint32_t)s.d1;
+    const uint32_t nt = (N + TS - 1) / TS;
+    const uint64_t pairs = (uint64_t)nt * (nt + 1) / 2;
+    auto k = k_square_swap<T, TS>;
+    int grid;
+    grid_for(ctx, k, 256, 0, pairs * (uint64_t)s.d3, t.grid_mult, &grid);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, data, N, nt, pairs, (uint64_t)s.d3);
+    IPT_LAUNCH_CHECK("k_square_swap");
+    return IPT_OK;
+}
+
+// ---------------------------------------------------------------- passes
+
+// Column gather pass new[i][j] = old[g(i,j)][j] over every slice.
+template <typename T, typename Ix, typename Map>
+static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostPlan& h,
+                         uint64_t d3, const Tune& t, hipStream_t st) {
+    const int64_t R = h.R, C = h.C, esz = sizeof(T);
+    const int64_t lds_max = t.lds_cap;
+    if (R * esz <= lds_max) {
+        // Widest power-of-two strip within the budget; wider than the budget
+        // only when even a 1-column strip does not fit it.
+        int64_t wmax = std::min<int64_t>(std::max(t.strip, 1), t.block);
+        int64_t cap = 1;
+        while (cap * 2 <= C && cap * 2 <= wmax) cap *= 2;
+        if (cap < C && cap * 2 <= wmax) cap *= 2;
+        int64_t W = cap;
+        while (W > 1 && R * W * esz > t.lds_bytes) W /= 2;
+        while (W > 1 && R * W * esz > lds_max) W /= 2;
+        const uint32_t nstrips = (uint32_t)((C + W - 1) / W);
+        const size_t lds = (size_t)(R * W * esz);
+        auto k = k_col_gather_lds<T, Ix, Map>;
+        ipt_status stt = allow_lds(k, lds);
+        if (stt != IPT_OK) return stt;
+        int grid;
+        grid_for(ctx, k, t.block, lds, d3 * nstrips, t.grid_mult, &grid);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(t.block), lds, st, data, p, d3, (uint32_t)W, nstrips);
+        IPT_LAUNCH_CHECK("k_col_gather_lds");
+        return IPT_OK;
+    }
+    // Columns taller than LDS: gather strips into scratch, copy back.
+    const uint32_t W = sizeof(T) == 4 ? 16 : 8;
+    const uint32_t nstrips = (uint32_t)((C + W - 1) / W);
+    const uint64_t items = d3 * nstrips;
+    const uint64_t per_item = (uint64_t)R * W * esz;
+    const uint64_t chunk = std::max<uint64_t>(1, kScratchCap / per_item);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    void* scr;
+    ipt_status stt = scratch_acquire(ctx, (size_t)(std::min(chunk, items) * per_item), st, &scr);
+    if (stt != IPT_OK) return stt;
+    for (uint64_t first = 0; first < items; first += chunk) {
+        const uint64_t n = std::min(chunk, items - first);
+        const uint64_t elems = n * R * W;
+        const int grid = (int)std::min<uint64_t>((elems + 255) / 256, (uint64_t)ncu_of(ctx) * 16);
+        hipLaunchKernelGGL((k_col_gather_to_scratch<T, Ix, Map>), dim3(grid), dim3(256), 0, st,
+                           data, (T*)scr, p, W, nstrips, first, n);
+        IPT_LAUNCH_CHECK("k_col_gather_to_scratch");
+        hipLaunchKernelGGL((k_col_scratch_back<T>), dim3(grid), dim3(256), 0, st, data,
+                           (const T*)scr, (uint64_t)R, (uint64_t)C, W, nstrips, first, n);
+        IPT_LAUNCH_CHECK("k_col_scratch_back");
+    }
+    return scratch_release(ctx, st);
+}
+
+// Row gather pass new[i][j] = old[i][s_i(j)] over every row of every slice.
+template <typename T, typename Ix, typename Map>
+static ipt_status row_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostPlan& h,
+                         uint64_t d3, const Tune& t, hipStream_t st) {
+    const int64_t R = h.R, C = h.C, esz = sizeof(T);
+    const int64_t row_bytes = C * esz;
+    const uint64_t nrows = d3 * (uint64_t)R;
+    if (row_bytes + 16 <= t.lds_cap) {
+        uint32_t per = (uint32_t)std::max<int64_t>(1, (t.lds_bytes - 16) / row_bytes);
+        per = (uint32_t)std::min<uint64_t>(per, nrows);
+        const size_t lds = (size_t)(per * row_bytes + 16);
+        auto k = k_row_gather_lds<T, Ix, Map>;
+        ipt_status stt = allow_lds(k, lds);
+        if (stt != IPT_OK) return stt;
+        int grid;
+        grid_for(ctx, k, t.block, lds, (nrows + per - 1) / per, t.grid_mult, &grid);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(t.block), lds, st, data, p, nrows, per,
+                           Div32::make((uint32_t)C));
+        IPT_LAUNCH_CHECK("k_row_gather_lds");
+        return IPT_OK;
+    }
+    // Rows longer than LDS: gather whole rows into scratch, copy back.
+    const uint64_t chunk = std::max<uint64_t>(1, kScratchCap / (uint64_t)row_bytes);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    void* scr;
+    ipt_status stt =
+        scratch_acquire(ctx, (size_t)(std::min(chunk, nrows) * (uint64_t)row_bytes), st, &scr);
+    if (stt != IPT_OK) return stt;
+    for (uint64_t g0 = 0; g0 < nrows; g0 += chunk) {
+        const uint64_t n = std::min(chunk, nrows - g0);
+        const uint64_t elems = n * (uint64_t)C;
+        const int grid = (int)std::min<uint64_t>((elems + 255) / 256, (uint64_t)ncu_of(ctx) * 16);
+        hipLaunchKernelGGL((k_row_gather_to_scratch<T, Ix, Map>), dim3(grid), dim3(256), 0, st,
+                           (const T*)data, (T*)scr, p, g0, n);
+        IPT_LAUNCH_CHECK("k_row_gather_to_scratch");
+        IPT_HIP_CHECK(hipMemcpyAsync(data + g0 * (uint64_t)C, scr, (size_t)(elems * esz),
+                                     hipMemcpyDeviceToDevice, st));
+    }
+    return scratch_release(ctx, st);
+}
+
+// ---------------------------------------------------------------- C2R / R2C
+
+// inplace::c2r::transpose (src/transpose.cu:30-47) on the view h (R x C).
+template <typename T, typename Ix>
+static ipt_status run_c2r_ix(DeviceCtx* ctx, T* data, const HostPlan& h, uint64_t d3,
+                             const Tune& t, hipStream_t st) {
+    const Plan<Ix> p = Plan<Ix>::make(h);
+    ipt_status r = IPT_OK;
+    if (h.c > 1) r = col_op<T, Ix, MapC2RPre>(ctx, data, p, h, d3, t, st);
+    if (r == IPT_OK) r = row_op<T, Ix, MapC2RShuffle>(ctx, data, p, h, d3, t, st);
+    if (r == IPT_OK) r = col_op<T, Ix, MapC2RPost>(ctx, data, p, h, d3, t, st);
+    return r;
+}
+
+// inplace::r2c::transpose (src/transpose.cu:70-89) on the view h (R x C).
+template <typename T, typename Ix>
+static ipt_status run_r2c_ix(DeviceCtx* ctx, T* data, const HostPlan& h, uint64_t d3,
+                             const Tune& t, hipStream_t st) {
+    const Plan<Ix> p = Plan<Ix>::make(h);
+    ipt_status r = col_op<T, Ix, MapR2CPre>(ctx, data, p, h, d3, t, st);
+    if (r == IPT_OK) r = row_op<T, Ix, MapR2CShuffle>(ctx, data, p, h, d3, t, st);
+    if (r == IPT_OK && h.c > 1) r = col_op<T, Ix, MapR2CPost>(ctx, data, p, h, d3, t, st);
+    return r;
+}
+
+template <typename T>
+static ipt_status run_decomp(DeviceCtx* ctx, T* data, const Shape& s, bool c2r, const Tune& t,
+                             hipStream_t st) {
+    // C2R transposes slices of R = d2 rows x C = d1 columns; R2C produces the
+    // same result with the view R = d1, C = d2 (the data is its C x R form).
+    const HostPlan h = c2r ? HostPlan::make(s.d2, s.d1) : HostPlan::make(s.d1, s.d2);
+    const uint64_t d3 = (uint64_t)s.d3;
+    if (h.fits_index32())
+        return c2r ? run_c2r_ix<T, uint32_t>(ctx, data, h, d3, t, st)
+                   : run_r2c_ix<T, uint32_t>(ctx, data, h, d3, t, st);
+    return c2r ? run_c2r_ix<T, uint64_t>(ctx, data, h, d3, t, st)
+               : run_r2c_ix<T, uint64_t>(ctx, data, h, d3, t, st);
+}
+
+template <typename T>
+static ipt_status run_typed(DeviceCtx* ctx, T* data, const Shape& s, int variant, const Tune& t,
+                            hipStream_t st) {
+    switch (variant) {
+    case IPT_VARIANT_SLICE_LDS: return run_slice_lds<T>(ctx, data, s, t, st);
+    case IPT_VARIANT_SQUARE_SWAP: return run_square<T>(ctx, data, s, t, st);
+    case IPT_VARIANT_C2R: return run_decomp<T>(ctx, data, s, true, t, st);
+    case IPT_VARIANT_R2C: return run_decomp<T>(ctx, data, s, false, t, st);
+    default: return IPT_ERR_INVALID_ARG;
+    }
+}
+
+ipt_status run_variant(DeviceCtx* ctx, void* data, const Shape& s, int variant, const Tune& t,
+                       hipStream_t stream) {
+    if (!variant_feasible(ctx, s, variant)) return IPT_ERR_UNSUPPORTED;
+    // A slice with one row or one column is its own transpose in memory.
+    if (s.d1 == 1 || s.d2 == 1) return IPT_OK;
+    if (s.esz == 4) return run_typed<uint32_t>(ctx, (uint32_t*)data, s, variant, t, stream);
+    return run_typed<uint64_t>(ctx, (uint64_t*)data, s, variant, t, stream);
+}
+
+template <typename T, typename Ix>
+static ipt_status run_pass_ix(DeviceCtx* ctx, T* data, const HostPlan& h, uint64_t d3, int map_id,
+                              const Tune& t, hipStream_t st) {
+    const Plan<Ix> p = Plan<Ix>::make(h);
+    switch (map_id) {
+    case MAP_C2R_PRE: return col_op<T, Ix, MapC2RPre>(ctx, data, p, h, d3, t, st);
+    case MAP_C2R_POST: return col_op<T, Ix, MapC2RPost>(ctx, data, p, h, d3, t, st);
+    case MAP_R2C_PRE: return col_op<T, Ix, MapR2CPre>(ctx, data, p, h, d3, t, st);
+    case MAP_R2C_POST: return col_op<T, Ix, MapR2CPost>(ctx, data, p, h, d3, t, st);
+    case MAP_C2R_SHUFFLE: return row_op<T, Ix, MapC2RShuffle>(ctx, data, p, h, d3, t, st);
+    case MAP_R2C_SHUFFLE: return row_op<T, Ix, MapR2CShuffle>(ctx, data, p, h, d3, t, st);
+    default: return IPT_ERR_INVALID_ARG;
+    }
+}
+
+ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, int64_t d3,
+                    int map_id, const Tune& t, hipStream_t stream) {
+    const HostPlan h = HostPlan::make(R, C);
+    if (h.fits_index32()) {
+        if (esz == 4) return run_pass_ix<uint32_t, uint32_t>(ctx, (uint32_t*)data, h, d3, map_id, t, stream);
+        return run_pass_ix<uint64_t, uint32_t>(ctx, (uint64_t*)data, h, d3, map_id, t, stream);
+    }
+    if (esz == 4) return run_pass_ix<uint32_t, uint64_t>(ctx, (uint32_t*)data, h, d3, map_id, t, stream);
+    return run_pass_ix<uint64_t, uint64_t>(ctx, (uint64_t*)data, h, d3, map_id, t, stream);
+}
+
+}  // namespace ipt

@@ -1,0 +1,83 @@
+// ipt_internal.h — host-side declarations shared by the library's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+
+#include "../../include/inplace_transpose.h"
+#include "ipt_plan.h"
+
+namespace ipt {
+
+// Per-device state: properties read on first use of the device (not at
+// static init, so a later hipSetDevice is honoured) and the cached scratch.
+struct DeviceCtx {
+    int dev = 0;
+    int ncu = 0;
+    int64_t lds_max = 0;
+    char arch[64] = {0};
+    std::mutex mu;  // guards scratch_* and serialises enqueues that use it
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    hipEvent_t scratch_event = nullptr;
+    bool scratch_event_valid = false;
+    hipStream_t scratch_stream = nullptr;
+};
+
+// One 213 problem: d3 slices of d2 rows x d1 columns, esz-byte elements.
+struct Shape {
+    int esz = 4;
+    int64_t d1 = 0, d2 = 0, d3 = 0;
+};
+
+// Resolved tuning knobs (see ipt_tune).
+struct Tune {
+    int lds_cap;    // hard LDS cap per workgroup: decides LDS tier vs fallback
+    int lds_bytes;  // LDS packing target per workgroup (<= lds_cap)
+    int block;      // threads per workgroup
+    int strip;      // max column-strip width (elements)
+    int grid_mult;  // workgroups per CU (x occupancy) for grid-stride kernels
+};
+
+constexpr int64_t kDefaultLds = 160 * 1024;       // gfx950 LDS per workgroup
+constexpr size_t kScratchCap = (size_t)256 << 20;  // fallback-tier scratch bound
+
+void set_error(const char* what, hipError_t e);
+ipt_status get_ctx(DeviceCtx** out);
+ipt_status scratch_acquire(DeviceCtx* ctx, size_t bytes, hipStream_t s, void** ptr);
+ipt_status scratch_release(DeviceCtx* ctx, hipStream_t s);
+ipt_status validate(const void* data, int esz, int64_t d1, int64_t d2, int64_t d3);
+Shape make_shape(int esz, int64_t d1, int64_t d2, int64_t d3);
+
+Tune resolve_tune(const DeviceCtx* ctx, const ipt_tune* t);
+bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant);
+int choose_variant(const DeviceCtx* ctx, const Shape& s);
+ipt_status run_variant(DeviceCtx* ctx, void* data, const Shape& s, int variant, const Tune& t,
+                       hipStream_t stream);
+ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, int64_t d3,
+                    int map_id, const Tune& t, hipStream_t stream);
+
+template <typename Ix>
+ipt_status eval_map_host(int map_id, const HostPlan& h, const int64_t* i, const int64_t* j,
+                         int64_t n, int64_t* out) {
+    Plan<Ix> p = Plan<Ix>::make(h);
+    for (int64_t t = 0; t < n; t++) {
+        if (i[t] < 0 || i[t] >= h.R || j[t] < 0 || j[t] >= h.C) return IPT_ERR_INVALID_ARG;
+        Ix ii = (Ix)i[t], jj = (Ix)j[t];
+        Ix r;
+        switch (map_id) {
+        case MAP_C2R_PRE: r = MapC2RPre::src(p, ii, jj); break;
+        case MAP_C2R_POST: r = MapC2RPost::src(p, ii, jj); break;
+        case MAP_R2C_PRE: r = MapR2CPre::src(p, ii, jj); break;
+        case MAP_R2C_POST: r = MapR2CPost::src(p, ii, jj); break;
+        case MAP_C2R_SHUFFLE: r = MapC2RShuffle::src(p, ii, jj); break;
+        default: r = MapR2CShuffle::src(p, ii, jj); break;
+        }
+        out[t] = (int64_t)r;
+    }
+    return IPT_OK;
+}
+
+}  // namespace ipt

@@ -1,0 +1,150 @@
+/*
+ * inplace_transpose.h — C ABI of the MI355X (gfx950) in-place transpose engine
+ * (libinplace_hip.so).
+ *
+ * Semantics follow the reference's oracle TensorUtil::seq_tt with permutation
+ * {1,0,2} (src/tensor_util.cpp:80-92): dims are listed fastest-first, so the
+ * buffer holds d3 contiguous slices, each a row-major matrix of d2 rows and d1
+ * columns, and each slice is replaced in place by its row-major d1 x d2
+ * transpose:
+ *
+ *     out[x1 + x0*d2 + x2*d1*d2] = in[x0 + x1*d1 + x2*d1*d2]
+ *
+ * Payloads are moved as raw bits (elem_bytes 4 or 8: the reference's int,
+ * long long, float and double instantiations, src/transpose.cu:126-129).
+ *
+ * Conventions (all entry points):
+ *   - `data` is device memory owned by the caller (hipMalloc'd or managed).
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *     Work is stream-ordered; the calls return once it is enqueued.
+ *   - Arguments are validated before any HIP call.  Nothing prints, throws or
+ *     exits; every failure is a returned ipt_status.
+ *   - Calls may come from several host threads; per-device state is guarded.
+ */
+#ifndef INPLACE_TRANSPOSE_H
+#define INPLACE_TRANSPOSE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    IPT_OK = 0,
+    IPT_ERR_INVALID_ARG = 1,  /* NULL pointer, non-positive dim, bad elem_bytes, ... */
+    IPT_ERR_UNSUPPORTED = 2,  /* a forced variant that cannot run this shape */
+    IPT_ERR_TOO_LARGE = 3,    /* sizes beyond the 64-bit index plan */
+    IPT_ERR_HIP = 4,          /* a HIP runtime call failed */
+    IPT_ERR_NO_DEVICE = 5,    /* no usable GPU */
+    IPT_ERR_OUT_OF_MEMORY = 6,/* library scratch could not be allocated */
+    IPT_ERR_IO = 7,           /* selector / tuning file missing or corrupt */
+    IPT_ERR_NOT_READY = 8     /* object used before it was built / trained */
+} ipt_status;
+
+/* Human-readable name of a status code (static storage). */
+const char* ipt_status_string(ipt_status s);
+
+/* Detail of the last IPT_ERR_HIP / IPT_ERR_OUT_OF_MEMORY on the calling
+ * thread (thread-local storage; empty when none). */
+const char* ipt_last_error(void);
+
+/* ---- transpose entry points ------------------------------------------- */
+
+/* Replaces inplace::transpose<T>(T* data, int d1, int d2, int d3)
+ * (include/transpose.h:17, src/transpose.cu:99-124): d3 slices of d2 rows x
+ * d1 columns, each transposed in place.  The variant is chosen by the
+ * dispatcher (learned selector when one is loaded, else the heuristic). */
+ipt_status ipt_transpose_213(void* data, int elem_bytes, int64_t d1, int64_t d2,
+                             int64_t d3, void* stream);
+
+/* Replaces inplace::c2r::transpose<T> (include/transpose.h:7,
+ * src/transpose.cu:30-47): forces the C2R decomposition; slices of d2 rows x
+ * d1 columns (kernel view R = d2, C = d1). */
+ipt_status ipt_c2r_transpose(void* data, int elem_bytes, int64_t d1, int64_t d2,
+                             int64_t d3, void* stream);
+
+/* Replaces inplace::r2c::transpose<T> (include/transpose.h:13,
+ * src/transpose.cu:70-89): forces the R2C decomposition with the reference's
+ * argument order: slices of d1 rows x d2 columns (kernel view R = d2, C = d1)
+ * are transposed to d2 rows x d1 columns. */
+ipt_status ipt_r2c_transpose(void* data, int elem_bytes, int64_t d1, int64_t d2,
+                             int64_t d3, void* stream);
+
+/* Replaces inplace::openmp::transpose(bool row_major, T* data, int m, int n,
+ * T* tmp) (include/openmp.h:7-8; declared but never implemented in the
+ * reference) with a GPU implementation: row-major m x n maps to
+ * (d1 = n, d2 = m, d3 = 1), column-major to (d1 = m, d2 = n, d3 = 1).  The
+ * reference's `tmp` argument is dropped: scratch is library-owned. */
+ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m,
+                         int64_t n, void* stream);
+
+/* ---- variants (used by the autotuner and the selector) ---------------- */
+
+typedef enum {
+    IPT_VARIANT_AUTO = 0,        /* dispatcher's choice */
+    IPT_VARIANT_SLICE_LDS = 1,   /* whole slice staged in LDS: one sweep */
+    IPT_VARIANT_SQUARE_SWAP = 2, /* square slices: tile-pair swap, one sweep */
+    IPT_VARIANT_C2R = 3,         /* C2R, view R = d2, C = d1 */
+    IPT_VARIANT_R2C = 4,         /* R2C, view R = d1, C = d2 */
+    IPT_VARIANT_COUNT = 5
+} ipt_variant;
+
+/* Tuning knobs; zero fields mean "library default". */
+typedef struct {
+    int32_t lds_bytes;    /* LDS cap per workgroup; smaller values push passes to
+                             the scratch tiers (tests use this) */
+    int32_t block;        /* threads per workgroup (power of two, 64..1024) */
+    int32_t strip;        /* column-strip width for LDS column passes (elements) */
+    int32_t grid_mult;    /* workgroups per CU for grid-stride kernels */
+} ipt_tune;
+
+/* Same semantics as ipt_transpose_213 with a forced variant and optional
+ * knobs (tune may be NULL).  IPT_ERR_UNSUPPORTED when the variant cannot run
+ * the shape (e.g. SQUARE_SWAP with d1 != d2). */
+ipt_status ipt_transpose_variant(void* data, int elem_bytes, int64_t d1, int64_t d2,
+                                 int64_t d3, int variant, const ipt_tune* tune,
+                                 void* stream);
+
+/* Whether `variant` can run the shape on the current device (1) or not (0).
+ * Host-only query: makes no HIP call apart from reading device properties. */
+int ipt_variant_feasible(int variant, int elem_bytes, int64_t d1, int64_t d2, int64_t d3);
+
+/* Plan inspection: the decomposition parameters the library uses for a view of
+ * R rows x C columns (c = gcd, a = R/c, b = C/c, k = a^-1 mod b,
+ * q = b^-1 mod a), and the variant the heuristic picks for (d1, d2, d3). */
+typedef struct {
+    int64_t R, C, c, a, b, k, q;
+    int32_t heuristic_variant;
+    int32_t index_bits;   /* 32 or 64: width of the in-slice index plan */
+} ipt_plan_info;
+ipt_status ipt_plan_query(int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
+                          ipt_plan_info* out);
+
+/* Host evaluation of the library's own index maps (the code the kernels run),
+ * for checking them against the oracle without a GPU.  map_id: 0 c2r pre,
+ * 1 c2r post, 2 r2c pre, 3 r2c post (column gathers: source row), 4 c2r
+ * shuffle, 5 r2c shuffle (row gathers: source column).  index_bits selects the
+ * 32- or 64-bit plan (32 only where the plan allows it).  Writes n results
+ * for the points (i[t], j[t]).  Pure host code. */
+ipt_status ipt_debug_eval_map(int map_id, int64_t R, int64_t C, int index_bits,
+                              const int64_t* i, const int64_t* j, int64_t n,
+                              int64_t* out);
+
+/* Run one pass of the decomposition on d3 slices viewed as R rows x C
+ * columns (map_id as for ipt_debug_eval_map: column gathers 0-3, row gathers
+ * 4-5).  Used for per-pass parity tests and per-kernel timing.  The
+ * reference's pass dispatchers this exposes: detail::rotate /
+ * detail::col_shuffle_fn (src/rotate.cu:237-270, src/col_shuffle.cu:30-42)
+ * and detail::shuffle_fn (src/shuffle.cu:66-76). */
+ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, int64_t d3,
+                              int map_id, const ipt_tune* tune, void* stream);
+
+/* Library build / device description, e.g. "gfx950 lds=163840 cus=256". */
+const char* ipt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
