@@ -24,10 +24,11 @@ import types
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libinplace_hip.so")
 
-VARIANT_AUTO, VARIANT_SLICE_LDS, VARIANT_SQUARE_SWAP, VARIANT_C2R, VARIANT_R2C = range(5)
+VARIANT_AUTO, VARIANT_SLICE_LDS, VARIANT_SQUARE_SWAP, VARIANT_C2R, VARIANT_R2C, VARIANT_GCS = range(6)
 VARIANT_NAMES = {VARIANT_AUTO: "auto", VARIANT_SLICE_LDS: "slice_lds",
-                 VARIANT_SQUARE_SWAP: "square_swap", VARIANT_C2R: "c2r", VARIANT_R2C: "r2c"}
-N_VARIANTS = 5
+                 VARIANT_SQUARE_SWAP: "square_swap", VARIANT_C2R: "c2r", VARIANT_R2C: "r2c",
+                 VARIANT_GCS: "gcs"}
+N_VARIANTS = 6
 
 # Map ids of ipt_debug_eval_map.
 MAP_C2R_PRE, MAP_C2R_POST, MAP_R2C_PRE, MAP_R2C_POST, MAP_C2R_SHUFFLE, MAP_R2C_SHUFFLE = range(6)
@@ -91,6 +92,10 @@ def lib():
     L.ipt_debug_eval_map.argtypes = [_int, _i64, _i64, _int, _vp, _vp, _i64, _vp]
     L.ipt_debug_run_pass.restype = _int
     L.ipt_debug_run_pass.argtypes = [_vp, _int, _i64, _i64, _i64, _int, ctypes.POINTER(Tune), _vp]
+    L.ipt_debug_copy.restype = _int
+    L.ipt_debug_copy.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
+    L.ipt_device_check.restype = _int
+    L.ipt_device_check.argtypes = []
     L.ipt_version.restype = ctypes.c_char_p
     L.ipt_version.argtypes = []
     _lib = L
@@ -192,6 +197,21 @@ def run_pass(data, R, C, d3, map_id, tune=None, stream=None):
     _check(lib().ipt_debug_run_pass(ptr, esz, R, C, d3, map_id, t, _stream_ptr(stream)),
            f"ipt_debug_run_pass({map_id})")
     return data
+
+
+def copy(dst, src, stream=None):
+    """Plain device copy src -> dst (calibration reference)."""
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() < nbytes:
+        raise TransposeError(1, "copy: destination too small")
+    _check(lib().ipt_debug_copy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()),
+                                nbytes, _stream_ptr(stream)), "ipt_debug_copy")
+    return dst
+
+
+def device_check():
+    """Synchronise and raise if a cooperative group barrier gave up waiting."""
+    _check(lib().ipt_device_check(), "ipt_device_check")
 
 
 def version():

@@ -54,6 +54,7 @@ ipt_status get_ctx(DeviceCtx** out) {
         return IPT_ERR_NO_DEVICE;
     }
     c->ncu = prop.multiProcessorCount;
+    c->sync.zero_at_rest = true;
     c->lds_max = (int64_t)prop.sharedMemPerBlock;
     snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
     *out = c.get();
@@ -61,49 +62,54 @@ ipt_status get_ctx(DeviceCtx** out) {
     return IPT_OK;
 }
 
-// Scratch is owned by the device context and shared by every call on that
-// device.  A user on another stream first waits for the previous user's last
-// kernel (event), so reuse is ordered; growth synchronises the device once.
-ipt_status scratch_acquire(DeviceCtx* ctx, size_t bytes, hipStream_t s, void** ptr) {
-    if (ctx->scratch_bytes < bytes) {
+// Grow-only arena.  Growth synchronises the device once (the old buffer may
+// still be in use by queued kernels) and is not allowed during stream capture.
+ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr) {
+    if (a.bytes < bytes) {
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) { set_error("hipDeviceSynchronize", e); return IPT_ERR_HIP; }
-        if (ctx->scratch) (void)hipFree(ctx->scratch);
-        ctx->scratch = nullptr;
-        ctx->scratch_bytes = 0;
-        size_t want = std::max(bytes, (size_t)64 << 20);
-        e = hipMalloc(&ctx->scratch, want);
+        if (a.ptr) (void)hipFree(a.ptr);
+        a.ptr = nullptr;
+        a.bytes = 0;
+        size_t want = a.zero_at_rest ? std::max(bytes, (size_t)1 << 16)
+                                     : std::max(bytes, (size_t)64 << 20);
+        e = hipMalloc(&a.ptr, want);
         if (e != hipSuccess) {
             (void)hipGetLastError();
-            e = hipMalloc(&ctx->scratch, bytes);
+            e = hipMalloc(&a.ptr, bytes);
             want = bytes;
         }
         if (e != hipSuccess) {
             (void)hipGetLastError();
-            ctx->scratch = nullptr;
-            set_error("hipMalloc(scratch)", e);
+            a.ptr = nullptr;
+            set_error("hipMalloc(arena)", e);
             return IPT_ERR_OUT_OF_MEMORY;
         }
-        ctx->scratch_bytes = want;
-        ctx->scratch_stream = s;
+        if (a.zero_at_rest) {
+            e = hipMemset(a.ptr, 0, want);
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+            if (e != hipSuccess) { set_error("hipMemset(arena)", e); return IPT_ERR_HIP; }
+        }
+        a.bytes = want;
+        a.last = s;
     }
-    if (ctx->scratch_event_valid && ctx->scratch_stream != s) {
-        hipError_t e = hipStreamWaitEvent(s, ctx->scratch_event, 0);
+    if (a.ev_valid && a.last != s) {
+        hipError_t e = hipStreamWaitEvent(s, a.ev, 0);
         if (e != hipSuccess) { set_error("hipStreamWaitEvent", e); return IPT_ERR_HIP; }
     }
-    *ptr = ctx->scratch;
+    *ptr = a.ptr;
     return IPT_OK;
 }
 
-ipt_status scratch_release(DeviceCtx* ctx, hipStream_t s) {
-    if (!ctx->scratch_event) {
-        hipError_t e = hipEventCreateWithFlags(&ctx->scratch_event, hipEventDisableTiming);
+ipt_status arena_release(Arena& a, hipStream_t s) {
+    if (!a.ev) {
+        hipError_t e = hipEventCreateWithFlags(&a.ev, hipEventDisableTiming);
         if (e != hipSuccess) { set_error("hipEventCreate", e); return IPT_ERR_HIP; }
     }
-    hipError_t e = hipEventRecord(ctx->scratch_event, s);
+    hipError_t e = hipEventRecord(a.ev, s);
     if (e != hipSuccess) { set_error("hipEventRecord", e); return IPT_ERR_HIP; }
-    ctx->scratch_event_valid = true;
-    ctx->scratch_stream = s;
+    a.ev_valid = true;
+    a.last = s;
     return IPT_OK;
 }
 
@@ -240,6 +246,35 @@ ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, 
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
     return run_pass(ctx, data, elem_bytes, R, C, d3, map_id, resolve_tune(ctx, tune),
                     (hipStream_t)stream);
+}
+
+ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src) return IPT_ERR_INVALID_ARG;
+    DeviceCtx* ctx;
+    ipt_status st = get_ctx(&ctx);
+    if (st != IPT_OK) return st;
+    return run_copy(ctx, dst, src, bytes, (hipStream_t)stream);
+}
+
+ipt_status ipt_device_check(void) {
+    DeviceCtx* ctx;
+    ipt_status st = get_ctx(&ctx);
+    if (st != IPT_OK) return st;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) { set_error("hipDeviceSynchronize", e); return IPT_ERR_HIP; }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (!ctx->sync.ptr) return IPT_OK;
+    uint32_t flag = 0;
+    e = hipMemcpy(&flag, (uint32_t*)ctx->sync.ptr + 2, sizeof(flag), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { set_error("hipMemcpy(sync flag)", e); return IPT_ERR_HIP; }
+    if (flag) {
+        // A group barrier gave up waiting: restore the zero-at-rest state.
+        e = hipMemset(ctx->sync.ptr, 0, ctx->sync.bytes);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        snprintf(g_last_error, sizeof(g_last_error), "group barrier watchdog tripped");
+        return IPT_ERR_HIP;
+    }
+    return IPT_OK;
 }
 
 const char* ipt_version(void) {

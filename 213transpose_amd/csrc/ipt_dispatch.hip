@@ -7,6 +7,8 @@
 //   C2R / R2C    the three-pass decomposition (src/transpose.cu:30-89), each
 //                pass a column or row gather; tiers per pass: LDS when the
 //                column strip / row fits, else scratch gather + copy back.
+//   GCS          a group of workgroups stages a slice in LDS, meets at a
+//                per-slice barrier, writes it transposed: one sweep.
 #include "ipt_internal.h"
 #include "ipt_kernels.h"
 
@@ -61,6 +63,58 @@ static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
     return s.d2 * slice_pitch(s.d1) * s.esz <= lds_max_of(ctx);
 }
 
+// ---------------------------------------------------------------- GCS geometry
+
+constexpr int kGcsBlock = 512;  // 8 waves per workgroup
+constexpr int kGcsRV = 8;       // 16-byte vectors per thread held in registers across the barrier
+
+struct GcsGeom {
+    int ts;           // tile side
+    bool vec;         // 16-byte global accesses
+    uint32_t ntj, ntiles;
+    uint32_t g;       // workgroups per slice
+    uint32_t K;       // tiles per member (max)
+    uint32_t KL;      // of which staged in LDS; the rest stay in registers
+    size_t lds;       // bytes per workgroup
+    bool ok;
+};
+
+// Geometry of the cooperative kernel.  Each member holds at most
+// KLmax tiles in LDS plus KRmax in registers.  When every slice fits in one
+// round of one workgroup per CU, slices are spread over all CUs (larger g,
+// less data per member); otherwise members are filled to capacity.
+static GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap) {
+    GcsGeom gg{};
+    const int64_t R = s.d2, C = s.d1, esz = s.esz;
+    gg.ts = esz == 4 ? 32 : 16;
+    gg.vec = (R * esz) % 16 == 0 && (C * esz) % 16 == 0;
+    const int64_t per = gg.vec ? 16 / esz : 1;
+    const int64_t vpt = (int64_t)gg.ts * gg.ts / per;
+    const int64_t tile_bytes = (int64_t)gg.ts * (gg.ts + 1) * esz;
+    const int64_t cap = std::min<int64_t>(lds_cap, lds_max_of(ctx)) - 64;  // static LDS headroom
+    const int64_t nti = (R + gg.ts - 1) / gg.ts, ntj = (C + gg.ts - 1) / gg.ts;
+    const int64_t ntiles = nti * ntj;
+    const int64_t klmax = cap / tile_bytes;
+    const int64_t krmax = std::min<int64_t>(klmax, (int64_t)kGcsRV * kGcsBlock / vpt);
+    gg.ok = klmax >= 1 && ntiles < ((int64_t)1 << 31) && R * C < ((int64_t)1 << 40);
+    if (!gg.ok) return gg;
+    const int64_t ncu = ncu_of(ctx);
+    const int64_t gmin = (ntiles + klmax + krmax - 1) / (klmax + krmax);
+    int64_t g = gmin;
+    if (s.d3 * gmin <= ncu) g = std::max<int64_t>(gmin, std::min<int64_t>(ncu / s.d3, ntiles));
+    const int64_t K = (ntiles + g - 1) / g;
+    gg.ntj = (uint32_t)ntj;
+    gg.ntiles = (uint32_t)ntiles;
+    gg.g = (uint32_t)g;
+    gg.K = (uint32_t)K;
+    gg.KL = (uint32_t)std::min(K, klmax);
+    gg.lds = (size_t)(gg.KL * tile_bytes);
+    // A group must never need more workgroups than one per CU (the residency
+    // every launch of this kernel is guaranteed).
+    gg.ok = g <= ncu && (uint64_t)s.d3 * (uint64_t)g < ((uint64_t)1 << 31);
+    return gg;
+}
+
 bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant) {
     switch (variant) {
     case IPT_VARIANT_AUTO: return true;
@@ -68,6 +122,7 @@ bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant) {
     case IPT_VARIANT_SQUARE_SWAP: return s.d1 == s.d2;
     case IPT_VARIANT_C2R:
     case IPT_VARIANT_R2C: return true;
+    case IPT_VARIANT_GCS: return gcs_geom(ctx, s, lds_max_of(ctx)).ok;
     default: return false;
     }
 }
@@ -77,6 +132,9 @@ bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant) {
 // d1 >= d2, else R2C with swapped arguments) and adds the one-sweep variants
 // first where they apply.
 int choose_variant(const DeviceCtx* ctx, const Shape& s) {
+    const int64_t slice_bytes = s.d1 * s.d2 * s.esz;
+    if (slice_bytes <= 16 * 1024 && slice_lds_fits(ctx, s)) return IPT_VARIANT_SLICE_LDS;
+    if (gcs_geom(ctx, s, lds_max_of(ctx)).ok) return IPT_VARIANT_GCS;
     if (slice_lds_fits(ctx, s)) return IPT_VARIANT_SLICE_LDS;
     if (s.d1 == s.d2) return IPT_VARIANT_SQUARE_SWAP;
     return s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
@@ -146,6 +204,49 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
     return IPT_OK;
 }
 
+// ---------------------------------------------------------------- GCS
+
+template <typename T, int TS, bool VEC>
+static ipt_status launch_gcs(DeviceCtx* ctx, T* data, const Shape& s, const GcsGeom& gg,
+                             const Tune& t, hipStream_t st) {
+    auto k = k_gcs<T, TS, VEC, kGcsBlock, kGcsRV>;
+    ipt_status stt = allow_lds(k, gg.lds);
+    if (stt != IPT_OK) return stt;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kGcsBlock, gg.lds) != hipSuccess ||
+        per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    const uint64_t slots = (uint64_t)s.d3 * gg.g;
+    const uint64_t resident = (uint64_t)per_cu * (uint64_t)ncu_of(ctx);
+    // Never more workgroups than can be resident at once: every member of a
+    // group must be running while the others wait for it.
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(slots, resident));
+    (void)t;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    void* sync;
+    stt = arena_acquire(ctx->sync, (size_t)(32 + 32 * (uint64_t)s.d3) * sizeof(uint32_t), st, &sync);
+    if (stt != IPT_OK) return stt;
+    uint32_t* ctl = (uint32_t*)sync;
+    if ((uint64_t)grid < gg.g) return IPT_ERR_UNSUPPORTED;  // a group must be co-resident
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, data, (uint32_t)s.d2,
+                       (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL, Div32::make(gg.ntj),
+                       gg.ntiles, ctl, ctl + 32);
+    IPT_LAUNCH_CHECK("k_gcs");
+    return arena_release(ctx->sync, st);
+}
+
+template <typename T>
+static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t, hipStream_t st) {
+    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap);
+    if (!gg.ok) return IPT_ERR_UNSUPPORTED;
+    const bool vec = gg.vec && ((uintptr_t)data % 16) == 0;
+    constexpr int TS = sizeof(T) == 4 ? 32 : 16;
+    return vec ? launch_gcs<T, TS, true>(ctx, data, s, gg, t, st)
+               : launch_gcs<T, TS, false>(ctx, data, s, gg, t, st);
+}
+
 // ---------------------------------------------------------------- passes
 
 // Column gather pass new[i][j] = old[g(i,j)][j] over every slice.
@@ -183,7 +284,7 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
     const uint64_t chunk = std::max<uint64_t>(1, kScratchCap / per_item);
     std::lock_guard<std::mutex> g(ctx->mu);
     void* scr;
-    ipt_status stt = scratch_acquire(ctx, (size_t)(std::min(chunk, items) * per_item), st, &scr);
+    ipt_status stt = arena_acquire(ctx->scratch, (size_t)(std::min(chunk, items) * per_item), st, &scr);
     if (stt != IPT_OK) return stt;
     for (uint64_t first = 0; first < items; first += chunk) {
         const uint64_t n = std::min(chunk, items - first);
@@ -196,7 +297,7 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
                            (const T*)scr, (uint64_t)R, (uint64_t)C, W, nstrips, first, n);
         IPT_LAUNCH_CHECK("k_col_scratch_back");
     }
-    return scratch_release(ctx, st);
+    return arena_release(ctx->scratch, st);
 }
 
 // Row gather pass new[i][j] = old[i][s_i(j)] over every row of every slice.
@@ -225,7 +326,7 @@ static ipt_status row_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
     std::lock_guard<std::mutex> g(ctx->mu);
     void* scr;
     ipt_status stt =
-        scratch_acquire(ctx, (size_t)(std::min(chunk, nrows) * (uint64_t)row_bytes), st, &scr);
+        arena_acquire(ctx->scratch, (size_t)(std::min(chunk, nrows) * (uint64_t)row_bytes), st, &scr);
     if (stt != IPT_OK) return stt;
     for (uint64_t g0 = 0; g0 < nrows; g0 += chunk) {
         const uint64_t n = std::min(chunk, nrows - g0);
@@ -237,7 +338,7 @@ static ipt_status row_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
         IPT_HIP_CHECK(hipMemcpyAsync(data + g0 * (uint64_t)C, scr, (size_t)(elems * esz),
                                      hipMemcpyDeviceToDevice, st));
     }
-    return scratch_release(ctx, st);
+    return arena_release(ctx->scratch, st);
 }
 
 // ---------------------------------------------------------------- C2R / R2C
@@ -287,6 +388,7 @@ static ipt_status run_typed(DeviceCtx* ctx, T* data, const Shape& s, int variant
     case IPT_VARIANT_SQUARE_SWAP: return run_square<T>(ctx, data, s, t, st);
     case IPT_VARIANT_C2R: return run_decomp<T>(ctx, data, s, true, t, st);
     case IPT_VARIANT_R2C: return run_decomp<T>(ctx, data, s, false, t, st);
+    case IPT_VARIANT_GCS: return run_gcs<T>(ctx, data, s, t, st);
     default: return IPT_ERR_INVALID_ARG;
     }
 }
@@ -324,6 +426,16 @@ ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, i
     }
     if (esz == 4) return run_pass_ix<uint32_t, uint64_t>(ctx, (uint32_t*)data, h, d3, map_id, t, stream);
     return run_pass_ix<uint64_t, uint64_t>(ctx, (uint64_t*)data, h, d3, map_id, t, stream);
+}
+
+ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (((uintptr_t)dst | (uintptr_t)src | bytes) % 16) return IPT_ERR_INVALID_ARG;
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0) return IPT_OK;
+    const int grid = (int)std::min<uint64_t>((n16 + 255) / 256, (uint64_t)ncu_of(ctx) * 16);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, (uint4*)dst, (const uint4*)src, n16);
+    IPT_LAUNCH_CHECK("k_copy16");
+    return IPT_OK;
 }
 
 }  // namespace ipt

@@ -11,19 +11,29 @@
 
 namespace ipt {
 
+// A device buffer owned by the library and shared by every call on one
+// device.  Users on different streams are ordered through `ev` (the next user
+// waits for the previous user's last kernel).  `zero_at_rest` buffers are
+// cleared when allocated and every kernel that uses them restores zeros.
+struct Arena {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    bool ev_valid = false;
+    hipStream_t last = nullptr;
+    bool zero_at_rest = false;
+};
+
 // Per-device state: properties read on first use of the device (not at
-// static init, so a later hipSetDevice is honoured) and the cached scratch.
+// static init, so a later hipSetDevice is honoured) and the cached arenas.
 struct DeviceCtx {
     int dev = 0;
     int ncu = 0;
     int64_t lds_max = 0;
     char arch[64] = {0};
-    std::mutex mu;  // guards scratch_* and serialises enqueues that use it
-    void* scratch = nullptr;
-    size_t scratch_bytes = 0;
-    hipEvent_t scratch_event = nullptr;
-    bool scratch_event_valid = false;
-    hipStream_t scratch_stream = nullptr;
+    std::mutex mu;  // guards the arenas and serialises enqueues that use them
+    Arena scratch;  // fallback-tier staging
+    Arena sync;     // group-barrier counters (zero at rest)
 };
 
 // One 213 problem: d3 slices of d2 rows x d1 columns, esz-byte elements.
@@ -46,8 +56,8 @@ constexpr size_t kScratchCap = (size_t)256 << 20;  // fallback-tier scratch boun
 
 void set_error(const char* what, hipError_t e);
 ipt_status get_ctx(DeviceCtx** out);
-ipt_status scratch_acquire(DeviceCtx* ctx, size_t bytes, hipStream_t s, void** ptr);
-ipt_status scratch_release(DeviceCtx* ctx, hipStream_t s);
+ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr);
+ipt_status arena_release(Arena& a, hipStream_t s);
 ipt_status validate(const void* data, int esz, int64_t d1, int64_t d2, int64_t d3);
 Shape make_shape(int esz, int64_t d1, int64_t d2, int64_t d3);
 
@@ -56,6 +66,7 @@ bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant);
 int choose_variant(const DeviceCtx* ctx, const Shape& s);
 ipt_status run_variant(DeviceCtx* ctx, void* data, const Shape& s, int variant, const Tune& t,
                        hipStream_t stream);
+ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st);
 ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, int64_t d3,
                     int map_id, const Tune& t, hipStream_t stream);
 

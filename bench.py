@@ -2,28 +2,39 @@
 
 Workload (one step): the 213 in-place permutes of [A=512][B=384][C=64] float32
 and [A=256][B=1000][C=17] float64 (dims fastest-first: d1=A, d2=B, d3=C), each
-a full transpose of every slice, inputs resident in HBM before the timed
-region.  value = sum over the step of 2 * bytes (each element read once and
-written once) / step time, in GB/s.
+a full transpose of every slice, inputs resident in device memory before the
+timed region.  value = sum over the step of 2 * bytes (each element read once
+and written once) / step time, in GB/s.
 
 The JSON line also carries:
   roofline      the dominant kernel of the step, timed alone with HIP events on
                 the stream it runs on: algorithmic bytes per launch (2 * tensor
                 bytes: one read and one write of every element) / average
-                launch duration, against the 8 TB/s HBM peak.
+                launch duration, against the 8 TB/s HBM peak.  `traffic` is the
+                measured bytes per launch of that kernel from two child
+                `rocprofv3 --pmc` runs (FETCH_SIZE, then WRITE_SIZE), each scaled
+                by what the same counter reads per byte for a plain copy kernel
+                of the same size.
+  copy_GBps     a plain device copy of the same tensors (2 * bytes / time): the
+                practical ceiling for any one-sweep transpose at this size.
   cpu_baseline  the oracle's OpenMP C2R/R2C path (a port: the reference's
                 OpenMP transpose is declared but never implemented) on the same
                 two tensors, on this host's cores, bounded to ~10 s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --profile ...   (workload + calibration copies only,
+                                        for running under a profiler)
 """
 import argparse
-import ctypes
+import glob
 import hashlib
 import importlib
 import json
 import os
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -35,6 +46,7 @@ WORKLOAD = [  # (d1, d2, d3, esz)
     (512, 384, 64, 4),
     (256, 1000, 17, 8),
 ]
+WORKLOAD_NAME = "213 in-place permute [512][384][64] f32 + [256][1000][17] f64"
 
 
 def parse():
@@ -43,6 +55,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE child runs")
+    ap.add_argument("--profile", action="store_true",
+                    help="only run workload steps and the calibration copies (for profilers)")
     return ap.parse_args()
 
 
@@ -54,20 +70,33 @@ def golden_sha(shape):
     return None
 
 
-def passes_of(ipt, d1, d2, d3, esz):
-    """The kernels one transpose launches: [(label, callable(tensor))]."""
+MAP_LABELS = {0: ("col:c2r_pre", "MapC2RPre"), 1: ("col:c2r_post", "MapC2RPost"),
+              2: ("col:r2c_pre", "MapR2CPre"), 3: ("col:r2c_post", "MapR2CPost"),
+              4: ("row:c2r_shuffle", "MapC2RShuffle"), 5: ("row:r2c_shuffle", "MapR2CShuffle")}
+
+
+def kernels_of(ipt, d1, d2, d3, esz):
+    """The kernels one transpose launches: (variant, [(label, fn(tensor), name match)])."""
     info = ipt.plan_query(esz, d1, d2, d3)
     v = info["heuristic_variant"]
-    if v in (ipt.VARIANT_SLICE_LDS, ipt.VARIANT_SQUARE_SWAP):
-        return v, [(ipt.VARIANT_NAMES[v], lambda t: ipt.transpose_variant(t, d1, d2, d3, v))]
+    tname = "unsigned int" if esz == 4 else "unsigned long"
+    single = {ipt.VARIANT_SLICE_LDS: "k_slice_lds", ipt.VARIANT_SQUARE_SWAP: "k_square_swap",
+              ipt.VARIANT_GCS: "k_gcs"}
+    if v in single:
+        fn = (lambda t: ipt.transpose_variant(t, d1, d2, d3, v))
+        return v, [(ipt.VARIANT_NAMES[v], fn, f"{single[v]}<{tname}")]
     R, C, c = info["R"], info["C"], info["c"]
     if v == ipt.VARIANT_C2R:
         maps = ([ipt.MAP_C2R_PRE] if c > 1 else []) + [ipt.MAP_C2R_SHUFFLE, ipt.MAP_C2R_POST]
     else:
         maps = [ipt.MAP_R2C_PRE, ipt.MAP_R2C_SHUFFLE] + ([ipt.MAP_R2C_POST] if c > 1 else [])
-    names = {0: "col:c2r_pre", 1: "col:c2r_post", 2: "col:r2c_pre", 3: "col:r2c_post",
-             4: "row:c2r_shuffle", 5: "row:r2c_shuffle"}
-    return v, [(names[m], (lambda t, m=m: ipt.run_pass(t, R, C, d3, m))) for m in maps]
+    ix = "unsigned int" if info["index_bits"] == 32 else "unsigned long"
+    out = []
+    for m in maps:
+        label, mname = MAP_LABELS[m]
+        out.append((label, (lambda t, m=m: ipt.run_pass(t, R, C, d3, m)),
+                    f"<{tname}, {ix}, ipt::{mname}>"))
+    return v, out
 
 
 def time_calls(torch, fn, tensor, reps):
@@ -86,7 +115,6 @@ def time_calls(torch, fn, tensor, reps):
 
 
 def cpu_baseline(oracle, budget_s=10.0):
-    import numpy as np
     threads = min(16, os.cpu_count() or 1)
     arrays = [oracle.random_bits(d1 * d2 * d3, esz, 7) for d1, d2, d3, esz in WORKLOAD]
     total_bytes = sum(2 * a.nbytes for a in arrays)
@@ -100,11 +128,70 @@ def cpu_baseline(oracle, budget_s=10.0):
         if time.perf_counter() - t_begin > budget_s or len(times) >= 30:
             break
     med = sorted(times)[len(times) // 2]
-    del np
     return {"value": round(total_bytes / med / 1e9, 3), "unit": "GB/s", "cores": threads,
             "kind": "port",
             "sample": f"{len(times)} full steps (both tensors) of the oracle's OpenMP C2R/R2C "
                       f"path, median step {med * 1e3:.1f} ms"}
+
+
+def make_tensors(torch):
+    tensors = []
+    for d1, d2, d3, esz in WORKLOAD:
+        dt = torch.int32 if esz == 4 else torch.int64
+        tensors.append(torch.arange(d1 * d2 * d3, dtype=dt, device="cuda"))
+    return tensors
+
+
+def profile_main(args, torch, ipt):
+    """Workload steps + calibration copies, nothing else (runs under rocprofv3)."""
+    tensors = make_tensors(torch)
+    sinks = [torch.empty_like(t) for t in tensors]
+    for _ in range(args.warmup + args.steps):
+        for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
+            ipt.transpose(t, d1, d2, d3)
+        for t, s in zip(tensors, sinks):
+            ipt.copy(s, t)
+    torch.cuda.synchronize()
+    print("profile run done", flush=True)
+
+
+def pmc_per_dispatch(db, counter):
+    """{kernel name: mean counter value per dispatch} from a rocpd database."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                       "where counter_name = ? group by kernel_name, dispatch_id", (counter,))
+    acc = {}
+    for name, _, val in rows:
+        acc.setdefault(name, []).append(val)
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def measure_traffic(match, shape_esz):
+    """Bytes per launch of the kernel whose name contains `match`, from
+    FETCH_SIZE and WRITE_SIZE, each scaled per byte by the copy kernel that
+    copies the same tensor (same bytes read and written per dispatch)."""
+    d1, d2, d3, esz = shape_esz
+    copy_bytes = d1 * d2 * d3 * esz
+    tname = "unsigned int" if esz == 4 else "unsigned long"
+    totals = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="ipt_pmc_")
+        cmd = ["timeout", "-s", "KILL", "120", "rocprofv3", "--pmc", counter, "-d", out, "-o",
+               "pmc", "--", sys.executable, os.path.abspath(__file__), "--profile", "--steps", "3",
+               "--warmup", "1"]
+        r = subprocess.run(cmd, cwd=tempfile.gettempdir(), capture_output=True, text=True)
+        dbs = glob.glob(os.path.join(out, "**", "*.db"), recursive=True)
+        if r.returncode not in (0, 139) or not dbs:
+            return None, f"rocprofv3 --pmc {counter} failed (rc={r.returncode})"
+        per = pmc_per_dispatch(dbs[0], counter)
+        kern = [v for k, v in per.items() if match in k]
+        # copies of both tensors appear: pick the one closest to this tensor's size
+        copies = sorted((v for k, v in per.items() if "k_copy16" in k),
+                        key=lambda v: abs(v * 1024 - copy_bytes))
+        if not copies or not kern:
+            return None, f"{counter}: kernel not found in profile ({tname})"
+        totals[counter] = kern[0] * (copy_bytes / copies[0])
+    return totals, None
 
 
 def main():
@@ -118,10 +205,10 @@ def main():
 
     ipt = importlib.import_module("213transpose_amd")
     torch.cuda.set_device(0)
-    tensors = []
-    for d1, d2, d3, esz in WORKLOAD:
-        dt = torch.int32 if esz == 4 else torch.int64
-        tensors.append(torch.arange(d1 * d2 * d3, dtype=dt, device="cuda"))
+    if args.profile:
+        profile_main(args, torch, ipt)
+        return
+    tensors = make_tensors(torch)
 
     # One checked transpose (reference iota input vs gen_ans digest), untimed.
     verified = True
@@ -131,6 +218,7 @@ def main():
         sha = hashlib.sha256(t.cpu().numpy().tobytes()).hexdigest()
         verified = verified and (sha == golden_sha((d1, d2, d3, esz)))
         ipt.transpose(t, d2, d1, d3)  # back to the original layout
+    ipt.device_check()
 
     def step():
         for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
@@ -153,20 +241,40 @@ def main():
     ms_per_step = ev0.elapsed_time(ev1) / args.steps
     step_bytes = sum(2 * t.numel() * t.element_size() for t in tensors)
     value = step_bytes / (ms_per_step * 1e-3) / 1e9
+    ipt.device_check()
 
     # Per-kernel timing (each kernel alone, events on the launch stream).
     kernels = []
     for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
-        variant, plist = passes_of(ipt, d1, d2, d3, esz)
+        variant, klist = kernels_of(ipt, d1, d2, d3, esz)
         alg = 2 * t.numel() * t.element_size()
-        for label, fn in plist:
+        for label, fn, match in klist:
             ms = time_calls(torch, fn, t, 20)
-            kernels.append({"kernel": label, "shape": [d1, d2, d3, esz],
+            kernels.append({"kernel": label, "shape": [d1, d2, d3, esz], "match": match,
                             "variant": ipt.VARIANT_NAMES[variant], "ms": round(ms, 5),
                             "alg_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1)})
     dom = max(kernels, key=lambda k: k["ms"])
+
+    # Practical ceiling: a plain copy of the same tensors.
+    sinks = [torch.empty_like(t) for t in tensors]
+    copy_ms = time_calls(torch, lambda _: [ipt.copy(s, t) for s, t in zip(sinks, tensors)],
+                         None, 20)
+    copy_gbps = step_bytes / (copy_ms * 1e-3) / 1e9
+    del sinks
+
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not args.no_traffic:
+        totals, err = measure_traffic(dom["match"], tuple(dom["shape"]))
+        if totals:
+            traffic = int(totals["FETCH_SIZE"] + totals["WRITE_SIZE"])
+            traffic_note = (f"FETCH {totals['FETCH_SIZE'] / 1e6:.1f} MB + WRITE "
+                            f"{totals['WRITE_SIZE'] / 1e6:.1f} MB per launch, copy-calibrated")
+        else:
+            traffic_note = err
+
     roofline = {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_note": traffic_note, "alg_bytes": dom["alg_bytes"],
                 "kernel": dom["kernel"], "shape": dom["shape"]}
 
     cpu = None
@@ -185,12 +293,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64 (raw 4/8-byte bit moves)",
+        "dtype": "u32+u64 (f32 / f64 payloads moved as raw bits)",
         "data": "synthetic (iota bit patterns), device-resident",
-        "config": {"workload": "213 in-place permute [512][384][64] f32 + [256][1000][17] f64",
+        "config": {"workload": WORKLOAD_NAME,
                    "residency": "MALL-resident (85 MB < 256 MB Infinity Cache)",
                    "parallelism": "single GPU"},
         "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
+        "copy_GBps": round(copy_gbps, 1),
         "verified_vs_gen_ans": verified,
         "wall_s": round(wall, 3),
         "roofline": roofline,

@@ -88,7 +88,11 @@ typedef enum {
     IPT_VARIANT_SQUARE_SWAP = 2, /* square slices: tile-pair swap, one sweep */
     IPT_VARIANT_C2R = 3,         /* C2R, view R = d2, C = d1 */
     IPT_VARIANT_R2C = 4,         /* R2C, view R = d1, C = d2 */
-    IPT_VARIANT_COUNT = 5
+    IPT_VARIANT_GCS = 5,         /* group-cooperative staged: a group of workgroups
+                                    stages a slice in LDS, then writes it
+                                    transposed; one sweep for slices up to the
+                                    GPU's resident LDS */
+    IPT_VARIANT_COUNT = 6
 } ipt_variant;
 
 /* Tuning knobs; zero fields mean "library default". */
@@ -140,6 +144,15 @@ ipt_status ipt_debug_eval_map(int map_id, int64_t R, int64_t C, int index_bits,
  * and detail::shuffle_fn (src/shuffle.cu:66-76). */
 ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, int64_t d3,
                               int map_id, const ipt_tune* tune, void* stream);
+
+/* Plain device copy of `bytes` (16-byte aligned pointers and size): the
+ * calibration reference for bandwidth and for the HBM traffic counters. */
+ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream);
+
+/* Synchronises the current device and reports whether any group barrier of
+ * the cooperative kernels gave up waiting (IPT_ERR_HIP, with detail in
+ * ipt_last_error; the barrier state is then reset).  IPT_OK otherwise. */
+ipt_status ipt_device_check(void);
 
 /* Library build / device description, e.g. "gfx950 lds=163840 cus=256". */
 const char* ipt_version(void);

@@ -53,11 +53,11 @@ def test_auto_dispatch_matches_oracle(ipt, oracle, esz):
         run_case(ipt, oracle, d1, d2, d3, esz, ipt.VARIANT_AUTO, 100 + k)
 
 
-@pytest.mark.parametrize("variant", ["c2r", "r2c", "slice_lds", "square_swap"])
+@pytest.mark.parametrize("variant", ["c2r", "r2c", "slice_lds", "square_swap", "gcs"])
 @pytest.mark.parametrize("esz", [4, 8])
 def test_each_variant_matches_oracle(ipt, oracle, variant, esz):
     v = {"c2r": ipt.VARIANT_C2R, "r2c": ipt.VARIANT_R2C, "slice_lds": ipt.VARIANT_SLICE_LDS,
-         "square_swap": ipt.VARIANT_SQUARE_SWAP}[variant]
+         "square_swap": ipt.VARIANT_SQUARE_SWAP, "gcs": ipt.VARIANT_GCS}[variant]
     ran = 0
     for k, (d1, d2, d3) in enumerate(SHAPES):
         if not ipt.feasible(v, esz, d1, d2, d3):
@@ -67,6 +67,31 @@ def test_each_variant_matches_oracle(ipt, oracle, variant, esz):
         run_case(ipt, oracle, d1, d2, d3, esz, v, 200 + k)
         ran += 1
     assert ran > 0
+
+
+@pytest.mark.parametrize("esz", [4, 8])
+def test_gcs_groups_and_barriers(ipt, oracle, esz):
+    """Cooperative kernel with large groups (small LDS caps), many slices per
+    launch (several rounds of slots), unaligned shapes (scalar path) and
+    back-to-back launches (counters restored to zero by each launch)."""
+    cases = [((512, 384, 64), None), ((256, 1000, 17), None), ((3000, 1000, 1), None),
+             ((97, 203, 40), ipt.Tune(lds_bytes=8192)), ((300, 100, 9), ipt.Tune(lds_bytes=4608)),
+             ((1000, 999, 1), ipt.Tune(lds_bytes=65536)), ((64, 2000, 3), ipt.Tune(lds_bytes=20000)),
+             ((33, 35, 500), ipt.Tune(lds_bytes=4608))]
+    ran = 0
+    for k, ((d1, d2, d3), tune) in enumerate(cases):
+        if not ipt.feasible(ipt.VARIANT_GCS, esz, d1, d2, d3):
+            continue
+        for rep in range(2):
+            try:
+                run_case(ipt, oracle, d1, d2, d3, esz, ipt.VARIANT_GCS, 600 + 10 * k + rep, tune=tune)
+            except ipt.TransposeError as e:
+                # a small LDS cap can make the group larger than one per CU
+                assert tune is not None and e.status == 2, e
+                break
+            ran += 1
+        ipt.device_check()
+    assert ran >= 8
 
 
 @pytest.mark.parametrize("esz", [4, 8])
