@@ -58,6 +58,18 @@ class PlanInfo(ctypes.Structure):
                [("heuristic_variant", ctypes.c_int32), ("index_bits", ctypes.c_int32)]
 
 
+class MatrixDesc(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("elem_bytes", ctypes.c_int32),
+                ("variant", ctypes.c_int32), ("d1", ctypes.c_int64), ("d2", ctypes.c_int64),
+                ("d3", ctypes.c_int64)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("count", "skipped", "n_instances", "n_big",
+                                               "n_slice_items", "n_gcs_items", "n_swap_items",
+                                               "grid", "lds_bytes")] + [("bytes", ctypes.c_uint64)]
+
+
 _i64, _vp, _int = ctypes.c_int64, ctypes.c_void_p, ctypes.c_int
 _lib = None
 
@@ -96,6 +108,18 @@ def lib():
     L.ipt_debug_copy.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
     L.ipt_device_check.restype = _int
     L.ipt_device_check.argtypes = []
+    L.ipt_batch_create.restype = _int
+    L.ipt_batch_create.argtypes = [ctypes.POINTER(MatrixDesc), _i64, ctypes.POINTER(Tune),
+                                   ctypes.POINTER(_vp)]
+    for name in ("ipt_batch_run", "ipt_batch_run_eager"):
+        getattr(L, name).restype = _int
+        getattr(L, name).argtypes = [_vp, _vp]
+    L.ipt_batch_get_info.restype = _int
+    L.ipt_batch_get_info.argtypes = [_vp, ctypes.POINTER(BatchInfo)]
+    L.ipt_batch_check.restype = _int
+    L.ipt_batch_check.argtypes = [_vp]
+    L.ipt_batch_destroy.restype = _int
+    L.ipt_batch_destroy.argtypes = [_vp]
     L.ipt_version.restype = ctypes.c_char_p
     L.ipt_version.argtypes = []
     _lib = L
@@ -207,6 +231,65 @@ def copy(dst, src, stream=None):
     _check(lib().ipt_debug_copy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()),
                                 nbytes, _stream_ptr(stream)), "ipt_debug_copy")
     return dst
+
+
+class Batch:
+    """A planned batch of uneven in-place transposes (ipt_batch_*).
+
+    items: iterable of (tensor, d1, d2, d3) or (tensor, d1, d2, d3, variant);
+    every tensor holds d3 slices of d2 x d1 and must stay alive while the
+    batch exists.  run() replays one captured hipGraph; run(graph=False)
+    launches the same work directly.
+    """
+
+    def __init__(self, items, tune=None):
+        items = list(items)
+        self._tensors = [it[0] for it in items]
+        descs = (MatrixDesc * max(1, len(items)))()
+        for k, it in enumerate(items):
+            t, d1, d2, d3 = it[:4]
+            ptr, esz = _args(t, d1, d2, d3)
+            descs[k] = MatrixDesc(ptr.value, esz, it[4] if len(it) > 4 else VARIANT_AUTO, d1, d2, d3)
+        self._h = ctypes.c_void_p()
+        tptr = ctypes.byref(tune) if tune is not None else None
+        _check(lib().ipt_batch_create(descs, len(items), tptr, ctypes.byref(self._h)),
+               "ipt_batch_create")
+
+    def run(self, graph=True, stream=None):
+        f = lib().ipt_batch_run if graph else lib().ipt_batch_run_eager
+        _check(f(self._h, _stream_ptr(stream)), "ipt_batch_run" if graph else "ipt_batch_run_eager")
+
+    def info(self):
+        inf = BatchInfo()
+        _check(lib().ipt_batch_get_info(self._h, ctypes.byref(inf)), "ipt_batch_get_info")
+        return {f: getattr(inf, f) for f, _ in BatchInfo._fields_}
+
+    def check(self):
+        """Synchronise and raise if a group barrier of this batch gave up waiting."""
+        _check(lib().ipt_batch_check(self._h), "ipt_batch_check")
+
+    def close(self):
+        if self._h:
+            _check(lib().ipt_batch_destroy(self._h), "ipt_batch_destroy")
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def uneven_shapes(count, lo=2, hi=4096, seed=0):
+    """The uneven-batch workload: `count` row-major matrices with m, n drawn
+    log-uniform in [lo, hi] and 4- or 8-byte elements (p = 1/2), seeded.
+    Returns [(m, n, esz)]; as a 213 problem each is (d1 = n, d2 = m, d3 = 1)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lm = rng.uniform(np.log(lo), np.log(hi + 1), size=(count, 2))
+    mn = np.minimum(np.floor(np.exp(lm)).astype(np.int64), hi)
+    esz = np.where(rng.random(count) < 0.5, 4, 8)
+    return [(int(m), int(n), int(e)) for (m, n), e in zip(mn, esz)]
 
 
 def device_check():

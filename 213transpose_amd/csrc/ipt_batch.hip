@@ -1,0 +1,333 @@
+// ipt_batch.hip — batched uneven transposes: one persistent launch over a
+// device work-item table, captured once into a hipGraph.
+//
+// No reference counterpart: the reference transposes one tensor per call
+// (src/transpose.cu:99-124), launching several kernels each time.  Here a
+// batch of instances of any shapes is planned once (items, barrier state,
+// launch geometry), and every run is one graph launch.
+#include "ipt_internal.h"
+#include "ipt_kernels.h"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace ipt {
+
+__global__ void __launch_bounds__(kBatchBlock)
+k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items, uint64_t nitems,
+        uint32_t* __restrict__ ctl, uint32_t* __restrict__ bars) {
+    IPT_DYN_LDS(unsigned char, lds);
+    for (uint64_t s = blockIdx.x; s < nitems; s += gridDim.x) {
+        const BatchItem it = items[s];
+        const BatchInst I = insts[it.inst];
+        if (I.esz == 4)
+            batch_item<uint32_t>(I, it, ctl, bars, reinterpret_cast<uint32_t*>(lds));
+        else
+            batch_item<uint64_t>(I, it, ctl, bars, reinterpret_cast<uint64_t*>(lds));
+        __syncthreads();  // LDS is reused by the next item
+    }
+}
+
+}  // namespace ipt
+
+using namespace ipt;
+
+struct ipt_batch_s {
+    DeviceCtx* ctx = nullptr;
+    BatchInst* d_insts = nullptr;
+    BatchItem* d_items = nullptr;
+    uint32_t* d_sync = nullptr;  // 32 control words + barrier state (zero at rest)
+    uint64_t sync_words = 0;
+    uint64_t nitems = 0;
+    size_t lds = 0;
+    int grid = 0;
+    struct Big {
+        void* data;
+        Shape s;
+        int variant;
+    };
+    std::vector<Big> big;  // instances run by the per-call launchers
+    Tune tune{};
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ipt_batch_info info{};
+};
+
+namespace {
+
+enum class Route { Skip, Slice, Gcs, Swap, Big };
+
+// Batch heuristic for one instance.  `forced` is the caller's variant
+// (IPT_VARIANT_AUTO = choose).
+Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t lds_cap, int grid,
+                   const Tune& t, int* big_variant) {
+    if (s.d1 == 1 || s.d2 == 1) return Route::Skip;
+    const int64_t slice_bytes = s.d2 * slice_pitch(s.d1) * s.esz;
+    const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false);
+    const bool gcs_ok = gg.ok && (int64_t)gg.g <= grid && gg.lds <= (size_t)lds_cap;
+    switch (forced) {
+    case IPT_VARIANT_SLICE_LDS: return slice_bytes <= lds_cap ? Route::Slice : Route::Big;
+    case IPT_VARIANT_GCS: return gcs_ok ? Route::Gcs : Route::Big;
+    case IPT_VARIANT_SQUARE_SWAP: return s.d1 == s.d2 ? Route::Swap : Route::Big;
+    case IPT_VARIANT_C2R:
+    case IPT_VARIANT_R2C: *big_variant = forced; return Route::Big;
+    default: break;
+    }
+    if (slice_bytes <= lds_cap) return Route::Slice;
+    if (gcs_ok) return Route::Gcs;
+    if (s.d1 == s.d2) return Route::Swap;
+    *big_variant = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
+    (void)t;
+    return Route::Big;
+}
+
+ipt_status enqueue(ipt_batch* b, hipStream_t st) {
+    if (b->nitems) {
+        hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBatchBlock), b->lds, st, b->d_insts,
+                           b->d_items, b->nitems, b->d_sync, b->d_sync + 32);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error("k_batch", e);
+            return IPT_ERR_HIP;
+        }
+    }
+    for (const auto& x : b->big) {
+        ipt_status r = run_variant(b->ctx, x.data, x.s, x.variant, b->tune, st);
+        if (r != IPT_OK) return r;
+    }
+    return IPT_OK;
+}
+
+void release(ipt_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->d_insts) (void)hipFree(b->d_insts);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_sync) (void)hipFree(b->d_sync);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const ipt_tune* tune,
+                            ipt_batch** out) {
+    if (!out || count < 0 || (count > 0 && !descs)) return IPT_ERR_INVALID_ARG;
+    *out = nullptr;
+    for (int64_t i = 0; i < count; i++) {
+        ipt_status st = validate(descs[i].data, descs[i].elem_bytes, descs[i].d1, descs[i].d2,
+                                 descs[i].d3);
+        if (st != IPT_OK) return st;
+        if (descs[i].variant < 0 || descs[i].variant >= IPT_VARIANT_COUNT)
+            return IPT_ERR_INVALID_ARG;
+    }
+    DeviceCtx* ctx;
+    ipt_status st = get_ctx(&ctx);
+    if (st != IPT_OK) return st;
+    ipt_batch* b = new (std::nothrow) ipt_batch;
+    if (!b) return IPT_ERR_OUT_OF_MEMORY;
+    b->ctx = ctx;
+    b->tune = resolve_tune(ctx, tune);
+    // One workgroup of kBatchBlock threads per CU at this LDS size (1 WG/CU
+    // is the residency the group barriers rely on).
+    const int64_t lds_cap = std::min<int64_t>(b->tune.lds_cap, lds_max_of(ctx)) - 64;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch, kBatchBlock, (size_t)lds_cap) !=
+            hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    const int grid = per_cu * ncu_of(ctx);
+
+    std::vector<BatchInst> insts;
+    std::vector<BatchItem> gcs_items, swap_items, slice_items;
+    std::vector<std::pair<uint64_t, BatchItem>> slice_sorted;
+    uint64_t bar_words_total = 0;
+    size_t lds_need = 0;
+    for (int64_t i = 0; i < count; i++) {
+        const ipt_matrix_desc& d = descs[i];
+        const Shape s = make_shape(d.elem_bytes, d.d1, d.d2, d.d3);
+        int big_variant = IPT_VARIANT_AUTO;
+        const Route p = plan_instance(ctx, s, d.variant, lds_cap, grid, b->tune, &big_variant);
+        if (p == Route::Skip) {
+            b->info.skipped++;
+            continue;
+        }
+        b->info.bytes += (uint64_t)s.d1 * s.d2 * s.d3 * s.esz;
+        if (p == Route::Big) {
+            if (big_variant == IPT_VARIANT_AUTO) big_variant = choose_variant(ctx, s);
+            const bool c2r = big_variant == IPT_VARIANT_C2R;
+            if ((big_variant == IPT_VARIANT_C2R || big_variant == IPT_VARIANT_R2C) &&
+                decomp_needs_scratch(s, c2r, b->tune)) {
+                release(b);
+                return IPT_ERR_UNSUPPORTED;  // scratch tiers are not used inside a batch
+            }
+            if (big_variant == IPT_VARIANT_GCS || !variant_feasible(ctx, s, big_variant)) {
+                release(b);
+                return IPT_ERR_UNSUPPORTED;
+            }
+            b->big.push_back({d.data, s, big_variant});
+            b->info.n_big++;
+            continue;
+        }
+        BatchInst I{};
+        I.data = (uint64_t)(uintptr_t)d.data;
+        I.R = (uint32_t)s.d2;
+        I.C = (uint32_t)s.d1;
+        I.esz = (uint32_t)s.esz;
+        const uint32_t idx = (uint32_t)insts.size();
+        if (p == Route::Slice) {
+            I.kind = BK_SLICE;
+            I.P = (uint32_t)slice_pitch(s.d1);
+            I.dS = Div32::make(I.R * I.C);
+            I.dC = Div32::make(I.C);
+            I.dR = Div32::make(I.R);
+            const uint64_t slice_bytes = (uint64_t)I.R * I.P * s.esz;
+            const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(
+                (uint64_t)b->tune.lds_bytes / slice_bytes, (uint64_t)lds_cap / slice_bytes));
+            lds_need = std::max<size_t>(lds_need, std::min<uint64_t>(per, (uint64_t)s.d3) * slice_bytes);
+            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl += per) {
+                BatchItem it{idx, (uint32_t)std::min<uint64_t>(per, (uint64_t)s.d3 - sl), sl};
+                slice_sorted.push_back({slice_bytes * it.sub, it});
+            }
+            b->info.n_slice_items += ((uint64_t)s.d3 + per - 1) / per;
+        } else if (p == Route::Gcs) {
+            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false);
+            const bool vec = gg.vec && ((uintptr_t)d.data % 16) == 0;
+            I.kind = vec ? BK_GCS_VEC : BK_GCS_SCALAR;
+            I.g = gg.g;
+            I.K = gg.K;
+            I.KL = gg.KL;
+            I.ntiles = gg.ntiles;
+            I.dntj = Div32::make(gg.ntj);
+            I.bar = bar_words_total;
+            bar_words_total += bar_words(gg.g) * (uint64_t)s.d3;
+            lds_need = std::max(lds_need, gg.lds);
+            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl++)
+                for (uint32_t m = 0; m < gg.g; m++) gcs_items.push_back({idx, m, sl});
+            b->info.n_gcs_items += (uint64_t)gg.g * s.d3;
+        } else {
+            I.kind = BK_SWAP;
+            const uint64_t TS = s.esz == 4 ? 64 : 32;
+            const uint64_t nt = ((uint64_t)s.d1 + TS - 1) / TS;
+            const uint64_t pairs = nt * (nt + 1) / 2;
+            if (pairs >= ((uint64_t)1 << 32)) {
+                release(b);
+                return IPT_ERR_TOO_LARGE;
+            }
+            lds_need = std::max<size_t>(lds_need, 2 * TS * (TS + 1) * s.esz);
+            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl++)
+                for (uint64_t t = 0; t < pairs; t++) swap_items.push_back({idx, (uint32_t)t, sl});
+            b->info.n_swap_items += pairs * s.d3;
+        }
+        insts.push_back(I);
+    }
+    // Heavy items first; the members of a group stay consecutive.
+    std::stable_sort(slice_sorted.begin(), slice_sorted.end(),
+                     [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<BatchItem> items;
+    items.reserve(gcs_items.size() + swap_items.size() + slice_sorted.size());
+    items.insert(items.end(), gcs_items.begin(), gcs_items.end());
+    items.insert(items.end(), swap_items.begin(), swap_items.end());
+    for (const auto& x : slice_sorted) items.push_back(x.second);
+
+    b->nitems = items.size();
+    b->lds = std::max<size_t>(lds_need, 16);
+    b->grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)grid, b->nitems));
+    b->info.count = count;
+    b->info.n_instances = (int64_t)insts.size();
+    b->info.grid = b->grid;
+    b->info.lds_bytes = (int64_t)b->lds;
+    hipError_t e = hipSuccess;
+    if (!insts.empty()) e = hipMalloc(&b->d_insts, insts.size() * sizeof(BatchInst));
+    if (e == hipSuccess && !items.empty()) e = hipMalloc(&b->d_items, items.size() * sizeof(BatchItem));
+    b->sync_words = 32 + bar_words_total;
+    const size_t sync_bytes = b->sync_words * sizeof(uint32_t);
+    if (e == hipSuccess) e = hipMalloc(&b->d_sync, sync_bytes);
+    if (e == hipSuccess) e = hipMemset(b->d_sync, 0, sync_bytes);
+    if (e == hipSuccess && !insts.empty())
+        e = hipMemcpy(b->d_insts, insts.data(), insts.size() * sizeof(BatchInst), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !items.empty())
+        e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess && b->lds > 64 * 1024)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("ipt_batch_create", e);
+        release(b);
+        return e == hipErrorOutOfMemory ? IPT_ERR_OUT_OF_MEMORY : IPT_ERR_HIP;
+    }
+    *out = b;
+    return IPT_OK;
+}
+
+ipt_status ipt_batch_run_eager(ipt_batch* b, void* stream) {
+    if (!b) return IPT_ERR_INVALID_ARG;
+    return enqueue(b, (hipStream_t)stream);
+}
+
+ipt_status ipt_batch_run(ipt_batch* b, void* stream) {
+    if (!b) return IPT_ERR_INVALID_ARG;
+    if (!b->exec) {
+        hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) {
+            set_error("hipStreamBeginCapture", e);
+            return IPT_ERR_HIP;
+        }
+        ipt_status st = enqueue(b, b->cap);
+        e = hipStreamEndCapture(b->cap, &b->graph);
+        if (st != IPT_OK) return st;
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            set_error("hipGraphInstantiate", e);
+            return IPT_ERR_HIP;
+        }
+    }
+    hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error("hipGraphLaunch", e);
+        return IPT_ERR_HIP;
+    }
+    return IPT_OK;
+}
+
+ipt_status ipt_batch_get_info(const ipt_batch* b, ipt_batch_info* info) {
+    if (!b || !info) return IPT_ERR_INVALID_ARG;
+    *info = b->info;
+    return IPT_OK;
+}
+
+ipt_status ipt_batch_check(ipt_batch* b) {
+    if (!b) return IPT_ERR_INVALID_ARG;
+    hipError_t e = hipDeviceSynchronize();
+    uint32_t flag = 0;
+    if (e == hipSuccess) e = hipMemcpy(&flag, b->d_sync + 2, sizeof(flag), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_error("ipt_batch_check", e);
+        return IPT_ERR_HIP;
+    }
+    if (flag) {
+        // A group barrier gave up waiting: restore the zero-at-rest state.
+        (void)hipMemset(b->d_sync, 0, (size_t)b->sync_words * sizeof(uint32_t));
+        (void)hipDeviceSynchronize();
+        set_error("group barrier watchdog tripped in a batch", hipSuccess);
+        return IPT_ERR_HIP;
+    }
+    return IPT_OK;
+}
+
+ipt_status ipt_batch_destroy(ipt_batch* b) {
+    if (!b) return IPT_ERR_INVALID_ARG;
+    (void)hipDeviceSynchronize();
+    release(b);
+    return IPT_OK;
+}
+
+}  // extern "C"

@@ -16,6 +16,17 @@
 
 namespace ipt {
 
+// ---------------------------------------------------------------------------
+// Plain copy (calibration reference for bandwidth and traffic counters):
+// 16 bytes per lane, grid-stride.  n16 = number of 16-byte words.
+__global__ void __launch_bounds__(256)
+k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+
 #define IPT_LAUNCH_CHECK(what)                          \
     do {                                                \
         hipError_t e_ = hipGetLastError();              \
@@ -34,8 +45,8 @@ namespace ipt {
         }                                               \
     } while (0)
 
-static int64_t lds_max_of(const DeviceCtx* ctx) { return ctx ? ctx->lds_max : kDefaultLds; }
-static int ncu_of(const DeviceCtx* ctx) { return ctx ? ctx->ncu : 256; }
+int64_t lds_max_of(const DeviceCtx* ctx) { return ctx ? ctx->lds_max : kDefaultLds; }
+int ncu_of(const DeviceCtx* ctx) { return ctx ? ctx->ncu : 256; }
 
 Tune resolve_tune(const DeviceCtx* ctx, const ipt_tune* t) {
     Tune r;
@@ -57,7 +68,7 @@ Tune resolve_tune(const DeviceCtx* ctx, const ipt_tune* t) {
 }
 
 // Odd LDS row pitch for the whole-slice kernel.
-static int64_t slice_pitch(int64_t C) { return C | 1; }
+int64_t slice_pitch(int64_t C) { return C | 1; }
 
 static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
     return s.d2 * slice_pitch(s.d1) * s.esz <= lds_max_of(ctx);
@@ -65,25 +76,11 @@ static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
 
 // ---------------------------------------------------------------- GCS geometry
 
-constexpr int kGcsBlock = 512;  // 8 waves per workgroup
-constexpr int kGcsRV = 8;       // 16-byte vectors per thread held in registers across the barrier
-
-struct GcsGeom {
-    int ts;           // tile side
-    bool vec;         // 16-byte global accesses
-    uint32_t ntj, ntiles;
-    uint32_t g;       // workgroups per slice
-    uint32_t K;       // tiles per member (max)
-    uint32_t KL;      // of which staged in LDS; the rest stay in registers
-    size_t lds;       // bytes per workgroup
-    bool ok;
-};
-
 // Geometry of the cooperative kernel.  Each member holds at most
 // KLmax tiles in LDS plus KRmax in registers.  When every slice fits in one
 // round of one workgroup per CU, slices are spread over all CUs (larger g,
 // less data per member); otherwise members are filled to capacity.
-static GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap) {
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread) {
     GcsGeom gg{};
     const int64_t R = s.d2, C = s.d1, esz = s.esz;
     gg.ts = esz == 4 ? 32 : 16;
@@ -101,7 +98,7 @@ static GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap) {
     const int64_t ncu = ncu_of(ctx);
     const int64_t gmin = (ntiles + klmax + krmax - 1) / (klmax + krmax);
     int64_t g = gmin;
-    if (s.d3 * gmin <= ncu) g = std::max<int64_t>(gmin, std::min<int64_t>(ncu / s.d3, ntiles));
+    if (spread && s.d3 * gmin <= ncu) g = std::max<int64_t>(gmin, std::min<int64_t>(ncu / s.d3, ntiles));
     const int64_t K = (ntiles + g - 1) / g;
     gg.ntj = (uint32_t)ntj;
     gg.ntiles = (uint32_t)ntiles;
@@ -163,6 +160,13 @@ static ipt_status allow_lds(K kernel, size_t lds) {
     return IPT_OK;
 }
 
+// Whether the C2R (c2r) or R2C decomposition of s falls to a scratch tier
+// (a row longer, or a column taller, than the LDS cap).
+bool decomp_needs_scratch(const Shape& s, bool c2r, const Tune& t) {
+    const int64_t R = c2r ? s.d2 : s.d1, C = c2r ? s.d1 : s.d2;
+    return C * s.esz + 16 > t.lds_cap || R * s.esz > t.lds_cap;
+}
+
 // ---------------------------------------------------------------- SLICE_LDS
 
 template <typename T>
@@ -199,7 +203,7 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
     auto k = k_square_swap<T, TS>;
     int grid;
     grid_for(ctx, k, 256, 0, pairs * (uint64_t)s.d3, t.grid_mult, &grid);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, data, N, nt, pairs, (uint64_t)s.d3);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, data, N, pairs, (uint64_t)s.d3);
     IPT_LAUNCH_CHECK("k_square_swap");
     return IPT_OK;
 }
@@ -226,7 +230,8 @@ static ipt_status launch_gcs(DeviceCtx* ctx, T* data, const Shape& s, const GcsG
     (void)t;
     std::lock_guard<std::mutex> lock(ctx->mu);
     void* sync;
-    stt = arena_acquire(ctx->sync, (size_t)(32 + 32 * (uint64_t)s.d3) * sizeof(uint32_t), st, &sync);
+    stt = arena_acquire(ctx->sync, (size_t)(32 + bar_words(gg.g) * (uint64_t)s.d3) * sizeof(uint32_t),
+                        st, &sync);
     if (stt != IPT_OK) return stt;
     uint32_t* ctl = (uint32_t*)sync;
     if ((uint64_t)grid < gg.g) return IPT_ERR_UNSUPPORTED;  // a group must be co-resident

@@ -70,6 +70,27 @@ ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hi
 ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, int64_t d3,
                     int map_id, const Tune& t, hipStream_t stream);
 
+constexpr int kGcsBlock = 512;  // 8 waves per workgroup
+constexpr int kGcsRV = 8;       // 16-byte vectors per thread held in registers across the barrier
+
+struct GcsGeom {
+    int ts;           // tile side
+    bool vec;         // 16-byte global accesses
+    uint32_t ntj, ntiles;
+    uint32_t g;       // workgroups per slice
+    uint32_t K;       // tiles per member (max)
+    uint32_t KL;      // of which staged in LDS; the rest stay in registers
+    size_t lds;       // bytes per workgroup
+    bool ok;
+};
+
+// spread: when every slice fits one round, use more members per slice.
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true);
+int64_t slice_pitch(int64_t C);
+int64_t lds_max_of(const DeviceCtx* ctx);
+int ncu_of(const DeviceCtx* ctx);
+bool decomp_needs_scratch(const Shape& s, bool c2r, const Tune& t);
+
 template <typename Ix>
 ipt_status eval_map_host(int map_id, const HostPlan& h, const int64_t* i, const int64_t* j,
                          int64_t n, int64_t* out) {

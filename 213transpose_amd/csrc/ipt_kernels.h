@@ -57,81 +57,81 @@ __device__ __forceinline__ void store_out(T* __restrict__ g, uint32_t n, F val) 
 }
 
 // ---------------------------------------------------------------------------
-// Plain copy (calibration reference for bandwidth and traffic counters):
-// 16 bytes per lane, grid-stride.  n16 = number of 16-byte words.
-__global__ void __launch_bounds__(256)
-k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+// Whole slices in LDS: one sweep.  `ns` consecutive slices of R x C starting
+// at `base` are read once into LDS with an odd row pitch P (so a column walk
+// touches distinct banks) and written back transposed.
+template <typename T>
+__device__ __forceinline__ void slice_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t P,
+                                           uint32_t ns, Div32 dS, Div32 dC, Div32 dR, T* lds) {
+    const uint32_t S = R * C;
+    const uint32_t SP = R * P;
+    const uint32_t n = ns * S;
+    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+        uint32_t sl = dS.div(e), r = e - sl * S;
+        uint32_t i = dC.div(r), j = r - i * C;
+        lds[sl * SP + i * P + j] = base[e];
+    }
+    __syncthreads();
+    for (uint32_t o = threadIdx.x; o < n; o += blockDim.x) {
+        uint32_t sl = dS.div(o), r = o - sl * S;
+        uint32_t jj = dR.div(r), ii = r - jj * R;  // output row jj (input column), col ii
+        base[o] = lds[sl * SP + ii * P + jj];
+    }
 }
 
-// ---------------------------------------------------------------------------
-// Whole slices in LDS: one sweep.  Each work item is `per` consecutive slices
-// of R x C; the slices are read once into LDS with an odd row pitch P (so a
-// column walk touches distinct banks) and written back transposed.
+// Each work item is `per` consecutive slices.
 template <typename T>
 __global__ void __launch_bounds__(1024)
 k_slice_lds(T* __restrict__ data, uint32_t R, uint32_t C, uint32_t P, uint64_t nslices,
             uint32_t per, Div32 dS, Div32 dC, Div32 dR) {
     IPT_DYN_LDS(T, lds);
-    const uint32_t S = R * C;
-    const uint32_t SP = R * P;
     for (uint64_t s0 = (uint64_t)blockIdx.x * per; s0 < nslices; s0 += (uint64_t)gridDim.x * per) {
         const uint32_t ns = (uint32_t)umin64(per, nslices - s0);
-        T* base = data + s0 * S;
-        const uint32_t n = ns * S;
-        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-            uint32_t sl = dS.div(e), r = e - sl * S;
-            uint32_t i = dC.div(r), j = r - i * C;
-            lds[sl * SP + i * P + j] = base[e];
-        }
-        __syncthreads();
-        for (uint32_t o = threadIdx.x; o < n; o += blockDim.x) {
-            uint32_t sl = dS.div(o), r = o - sl * S;
-            uint32_t jj = dR.div(r), ii = r - jj * R;  // output row jj (input column), col ii
-            base[o] = lds[sl * SP + ii * P + jj];
-        }
+        slice_body(data + s0 * R * C, R, C, P, ns, dS, dC, dR, lds);
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // Square slices N x N: swap-transpose tile pairs (I,J) / (J,I), I <= J.
-// One sweep; both tiles are staged in LDS before either is written.
+// One sweep; both tiles are staged in LDS (2 * TS * (TS+1) elements at lds)
+// before either is written.  Pair t = J(J+1)/2 + I.
+template <typename T, int TS>
+__device__ __forceinline__ void swap_body(T* __restrict__ base, uint32_t N, uint64_t t, T* lds) {
+    T* A = lds;
+    T* B = lds + TS * (TS + 1);
+    constexpr uint32_t TP = TS + 1;
+    const uint32_t tx = threadIdx.x % TS, ty = threadIdx.x / TS;
+    const uint32_t rows = blockDim.x / TS;
+    uint64_t J = (uint64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (J * (J + 1) / 2 > t) J--;
+    while ((J + 1) * (J + 2) / 2 <= t) J++;
+    const uint64_t I = t - J * (J + 1) / 2;
+    const uint32_t r0 = (uint32_t)I * TS, c0 = (uint32_t)J * TS;
+    for (uint32_t y = ty; y < TS; y += rows) {
+        uint32_t r = r0 + y, c = c0 + tx;
+        if (r < N && c < N) A[y * TP + tx] = base[(uint64_t)r * N + c];
+        uint32_t r2 = c0 + y, c2 = r0 + tx;
+        if (I != J && r2 < N && c2 < N) B[y * TP + tx] = base[(uint64_t)r2 * N + c2];
+    }
+    __syncthreads();
+    for (uint32_t y = ty; y < TS; y += rows) {
+        // tile (I,J) <- transpose of tile (J,I); diagonal: of itself.
+        uint32_t r = r0 + y, c = c0 + tx;
+        if (r < N && c < N) base[(uint64_t)r * N + c] = (I == J) ? A[tx * TP + y] : B[tx * TP + y];
+        uint32_t r2 = c0 + y, c2 = r0 + tx;
+        if (I != J && r2 < N && c2 < N) base[(uint64_t)r2 * N + c2] = A[tx * TP + y];
+    }
+}
+
 template <typename T, int TS>
 __global__ void __launch_bounds__(256)
-k_square_swap(T* __restrict__ data, uint32_t N, uint32_t ntiles, uint64_t pairs, uint64_t d3) {
-    __shared__ T A[TS][TS + 1];
-    __shared__ T B[TS][TS + 1];
-    const uint32_t tx = threadIdx.x % TS, ty = threadIdx.x / TS;
-    constexpr uint32_t ROWS = 256 / TS;
+k_square_swap(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3) {
+    __shared__ T tiles[2 * TS * (TS + 1)];
     const uint64_t total = pairs * d3;
     for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const uint64_t sl = w / pairs;
-        const uint64_t t = w - sl * pairs;
-        // t = J(J+1)/2 + I with I <= J.
-        uint64_t J = (uint64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while (J * (J + 1) / 2 > t) J--;
-        while ((J + 1) * (J + 2) / 2 <= t) J++;
-        const uint64_t I = t - J * (J + 1) / 2;
-        (void)ntiles;
-        T* base = data + sl * (uint64_t)N * N;
-        const uint32_t r0 = (uint32_t)I * TS, c0 = (uint32_t)J * TS;
-        for (uint32_t y = ty; y < TS; y += ROWS) {
-            uint32_t r = r0 + y, c = c0 + tx;
-            if (r < N && c < N) A[y][tx] = base[(uint64_t)r * N + c];
-            uint32_t r2 = c0 + y, c2 = r0 + tx;
-            if (I != J && r2 < N && c2 < N) B[y][tx] = base[(uint64_t)r2 * N + c2];
-        }
-        __syncthreads();
-        for (uint32_t y = ty; y < TS; y += ROWS) {
-            // tile (I,J) <- transpose of tile (J,I); diagonal: of itself.
-            uint32_t r = r0 + y, c = c0 + tx;
-            if (r < N && c < N) base[(uint64_t)r * N + c] = (I == J) ? A[tx][y] : B[tx][y];
-            uint32_t r2 = c0 + y, c2 = r0 + tx;
-            if (I != J && r2 < N && c2 < N) base[(uint64_t)r2 * N + c2] = A[tx][y];
-        }
+        swap_body<T, TS>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles);
         __syncthreads();
     }
 }
@@ -268,9 +268,8 @@ namespace ipt {
 //
 // Sync state (device memory, zero at rest; each launch restores it):
 //   ctl[2] watchdog flag (sticky)
-//   cnt[32*S] arrivals, cnt[32*S + 16] departures of slice S (one 128-byte
-//   line per slice, so groups never contend on a line).  The last member to
-//   depart zeroes both.
+//   cnt[bar_words(g) * S ...] the barrier lines of slice S (see
+//   group_arrive_wait); no two slices share a line.
 // The spin is bounded: after ~2^26 polls the member records ctl[2] and
 // proceeds, so a broken invariant shows up as an error, never as a hang.
 //
@@ -300,6 +299,35 @@ __device__ __forceinline__ bool spin_until(uint32_t* ctr, uint32_t target) {
         if (++polls > kSpinLimit) return false;
     }
     return true;
+}
+
+// Group barrier of g members over kBarLines 128-byte lines (one slice):
+//   line 0: word 0 top counter (sub-counters completed), word 16 departures
+//   line 1 + q: word 0 arrivals of sub-group q (members m with m % nsub == q)
+// With nsub = min(g, kBarLines - 1) sub-counters, at most ceil(g / nsub)
+// members contend on a line.  The last arriver of a sub-group bumps the top
+// counter; members wait for the top counter to reach nsub.  The last member
+// to depart zeroes every word it used (zero at rest).
+constexpr uint32_t kBarLines = 17;
+
+// Words of barrier state per slice for a group of g.
+IPT_HD uint64_t bar_words(uint32_t g) { return 32ull * (1 + (g < kBarLines - 1 ? g : kBarLines - 1)); }
+
+__device__ __forceinline__ bool group_arrive_wait(uint32_t* bar, uint32_t m, uint32_t g) {
+    const uint32_t nsub = min(g, kBarLines - 1);
+    const uint32_t q = m % nsub;
+    const uint32_t members = g / nsub + (q < g % nsub ? 1u : 0u);
+    if (atomic_add_agent(&bar[32 * (1 + q)], 1u) == members - 1) atomic_add_agent(&bar[0], 1u);
+    return spin_until(&bar[0], nsub);
+}
+
+__device__ __forceinline__ void group_depart(uint32_t* bar, uint32_t g) {
+    if (atomic_add_agent(&bar[16], 1u) == g - 1) {
+        const uint32_t nsub = min(g, kBarLines - 1);
+        for (uint32_t q = 0; q < nsub; q++) atomic_store_agent(&bar[32 * (1 + q)], 0u);
+        atomic_store_agent(&bar[0], 0u);
+        atomic_store_agent(&bar[16], 0u);
+    }
 }
 
 template <typename T, int TS, bool VEC>
@@ -370,85 +398,147 @@ __device__ __forceinline__ void gcs_store(T* __restrict__ base, const T* lds, ui
     }
 }
 
-// GCS kernel.  Member m of slice S owns tiles t = m + k*g, k < nk; the first
-// kl of them are staged in LDS, the remaining nk - kl (at most RV * BLOCK /
-// VPT) stay in registers across the group barrier and go through LDS after
-// the LDS-staged tiles have been written.
+// One GCS slot: member m of the group of g that transposes the slice at
+// `base`.  Member m owns tiles t = m + k*g, k < nk; the first kl of them are
+// staged in LDS, the remaining nk - kl (at most RV * BLOCK / VPT) stay in
+// registers across the group barrier and go through LDS after the
+// LDS-staged tiles have been written.  `bar` is the slice's barrier state.
+template <typename T, int TS, bool VEC, int BLOCK, int RV>
+__device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
+                                         uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
+                                         uint32_t m, uint32_t* bar, uint32_t* ctl, T* lds) {
+    using G = GcsTile<T, TS, VEC>;
+    using V = typename G::V;
+    constexpr uint32_t U = 32;  // LDS-bound vectors in flight per thread
+    const uint32_t nk = m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0;
+    const uint32_t kl = min(nk, KL), kr = nk - kl;
+
+    // register-held tiles: issued first, kept across the barrier
+    V rreg[RV];
+    const uint32_t rtotal = kr * G::VPT;
+#pragma unroll
+    for (uint32_t u = 0; u < RV; u++) {
+        const uint32_t idx = u * BLOCK + threadIdx.x;
+        if (idx < rtotal) {
+            uint64_t off;
+            if (gcs_src<T, TS, VEC>(m + (kl + idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off))
+                rreg[u] = *reinterpret_cast<const V*>(base + off);
+        }
+    }
+    // LDS-staged tiles, U vectors per thread in flight per batch
+    const uint32_t ltotal = kl * G::VPT;
+    for (uint32_t b0 = 0; b0 < ltotal; b0 += U * BLOCK) {
+        V reg[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t idx = b0 + u * BLOCK + threadIdx.x;
+            if (idx < ltotal) {
+                uint64_t off;
+                if (gcs_src<T, TS, VEC>(m + (idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off))
+                    reg[u] = *reinterpret_cast<const V*>(base + off);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t idx = b0 + u * BLOCK + threadIdx.x;
+            if (idx < ltotal) gcs_put<T, TS, VEC>(lds + (idx / G::VPT) * G::TILE, idx % G::VPT, reg[u]);
+        }
+    }
+    __syncthreads();
+    if (g > 1) {
+        if (threadIdx.x == 0 && !group_arrive_wait(bar, m, g)) atomic_store_agent(&ctl[2], 1u);
+        __syncthreads();
+    }
+    gcs_store<T, TS, VEC, BLOCK>(base, lds, R, C, dntj, m, g, 0, kl);
+    if (kr) {
+        __syncthreads();
+#pragma unroll
+        for (uint32_t u = 0; u < RV; u++) {
+            const uint32_t idx = u * BLOCK + threadIdx.x;
+            if (idx < rtotal) gcs_put<T, TS, VEC>(lds + (idx / G::VPT) * G::TILE, idx % G::VPT, rreg[u]);
+        }
+        __syncthreads();
+        gcs_store<T, TS, VEC, BLOCK>(base, lds, R, C, dntj, m, g, kl, kr);
+    }
+    if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
+}
+
+// GCS kernel: slot s is member s % g of slice s / g.
 template <typename T, int TS, bool VEC, int BLOCK, int RV>
 __global__ void __launch_bounds__(BLOCK)
 k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
       uint32_t KL, Div32 dntj, uint32_t ntiles, uint32_t* __restrict__ ctl,
       uint32_t* __restrict__ cnt) {
-    using G = GcsTile<T, TS, VEC>;
-    using V = typename G::V;
-    constexpr uint32_t U = 16;  // LDS-bound vectors in flight per thread
     IPT_DYN_LDS(T, lds);
     const uint64_t total = nslices * g;
     const uint64_t slice_elems = (uint64_t)R * C;
     for (uint64_t slot = blockIdx.x; slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
-        T* base = data + S * slice_elems;
-        const uint32_t nk = m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0;
-        const uint32_t kl = min(nk, KL), kr = nk - kl;
-
-        // register-held tiles: issued first, kept across the barrier
-        V rreg[RV];
-        const uint32_t rtotal = kr * G::VPT;
-#pragma unroll
-        for (uint32_t u = 0; u < RV; u++) {
-            const uint32_t idx = u * BLOCK + threadIdx.x;
-            if (idx < rtotal) {
-                uint64_t off;
-                if (gcs_src<T, TS, VEC>(m + (kl + idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off))
-                    rreg[u] = *reinterpret_cast<const V*>(base + off);
-            }
-        }
-        // LDS-staged tiles, U vectors per thread in flight per batch
-        const uint32_t ltotal = kl * G::VPT;
-        for (uint32_t b0 = 0; b0 < ltotal; b0 += U * BLOCK) {
-            V reg[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; u++) {
-                const uint32_t idx = b0 + u * BLOCK + threadIdx.x;
-                if (idx < ltotal) {
-                    uint64_t off;
-                    if (gcs_src<T, TS, VEC>(m + (idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off))
-                        reg[u] = *reinterpret_cast<const V*>(base + off);
-                }
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < U; u++) {
-                const uint32_t idx = b0 + u * BLOCK + threadIdx.x;
-                if (idx < ltotal) gcs_put<T, TS, VEC>(lds + (idx / G::VPT) * G::TILE, idx % G::VPT, reg[u]);
-            }
-        }
-        __syncthreads();
-        if (g > 1) {
-            if (threadIdx.x == 0) {
-                atomic_add_agent(&cnt[32 * S], 1u);
-                if (!spin_until(&cnt[32 * S], g)) atomic_store_agent(&ctl[2], 1u);
-            }
-            __syncthreads();
-        }
-        gcs_store<T, TS, VEC, BLOCK>(base, lds, R, C, dntj, m, g, 0, kl);
-        if (kr) {
-            __syncthreads();
-#pragma unroll
-            for (uint32_t u = 0; u < RV; u++) {
-                const uint32_t idx = u * BLOCK + threadIdx.x;
-                if (idx < rtotal) gcs_put<T, TS, VEC>(lds + (idx / G::VPT) * G::TILE, idx % G::VPT, rreg[u]);
-            }
-            __syncthreads();
-            gcs_store<T, TS, VEC, BLOCK>(base, lds, R, C, dntj, m, g, kl, kr);
-        }
-        if (g > 1 && threadIdx.x == 0) {
-            if (atomic_add_agent(&cnt[32 * S + 16], 1u) == g - 1) {
-                atomic_store_agent(&cnt[32 * S], 0u);
-                atomic_store_agent(&cnt[32 * S + 16], 0u);
-            }
-        }
+        gcs_body<T, TS, VEC, BLOCK, RV>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
+                                        cnt + bar_words(g) * S, ctl, lds);
         __syncthreads();  // LDS is reused by the next slot
+    }
+}
+
+}  // namespace ipt
+
+namespace ipt {
+
+// ---------------------------------------------------------------------------
+// Batched uneven transposes: one persistent launch over a table of work
+// items.  Each instance (d3 slices of R x C, 4- or 8-byte elements) is cut
+// into items of one kind:
+//   BK_SLICE  `sub` consecutive slices staged whole in LDS (slice_body)
+//   BK_GCS    member `sub` of the group of g that owns one slice (gcs_body);
+//             the g members of a slice occupy consecutive items
+//   BK_SWAP   tile pair `sub` of one square slice (swap_body)
+// Workgroup b takes items b, b + grid, ... (static round-robin), so the
+// deadlock argument of k_gcs holds unchanged (every g <= grid, other items
+// never wait).
+enum BatchKind : uint32_t { BK_SLICE = 0, BK_GCS_VEC = 1, BK_GCS_SCALAR = 2, BK_SWAP = 3 };
+
+struct BatchInst {
+    uint64_t data;        // device address of slice 0
+    uint64_t bar;         // word offset of slice 0's barrier state (BK_GCS_*)
+    uint32_t R, C, esz, kind;
+    uint32_t g, K, KL, ntiles;   // BK_GCS_*
+    Div32 dntj;                  // BK_GCS_*
+    uint32_t P;                  // BK_SLICE: LDS row pitch
+    Div32 dS, dC, dR;            // BK_SLICE
+};
+
+struct BatchItem {
+    uint32_t inst;
+    uint32_t sub;    // slices in the item / member / tile pair
+    uint64_t slice;  // first slice
+};
+
+constexpr int kBatchBlock = 512;
+
+template <typename T>
+__device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& it, uint32_t* ctl,
+                                           uint32_t* bars, T* lds) {
+    constexpr int TS_GCS = sizeof(T) == 4 ? 32 : 16;
+    constexpr int TS_SWAP = sizeof(T) == 4 ? 64 : 32;
+    T* base = reinterpret_cast<T*>(I.data) + it.slice * (uint64_t)I.R * I.C;
+    switch (I.kind) {
+    case BK_SLICE:
+        slice_body<T>(base, I.R, I.C, I.P, it.sub, I.dS, I.dC, I.dR, lds);
+        break;
+    case BK_GCS_VEC:
+        gcs_body<T, TS_GCS, true, kBatchBlock, 8>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles,
+                                                  it.sub, bars + I.bar + bar_words(I.g) * it.slice,
+                                                  ctl, lds);
+        break;
+    case BK_GCS_SCALAR:
+        gcs_body<T, TS_GCS, false, kBatchBlock, 8>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+                                                   I.ntiles, it.sub,
+                                                   bars + I.bar + bar_words(I.g) * it.slice, ctl, lds);
+        break;
+    default:
+        swap_body<T, TS_SWAP>(base, I.R, it.sub, lds);
+        break;
     }
 }
 

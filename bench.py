@@ -17,6 +17,11 @@ The JSON line also carries:
                 of the same size.
   copy_GBps     a plain device copy of the same tensors (2 * bytes / time): the
                 practical ceiling for any one-sweep transpose at this size.
+  batched       the uneven-batch workload (4096 row-major matrices, m and n
+                log-uniform in [2, 4096], 4- or 8-byte elements, seed 4): one
+                graph launch of the planned batch vs the plain per-call
+                heuristic loop (one ipt_transpose_213 per matrix); speedup =
+                t(per-call) / t(batch).
   cpu_baseline  the oracle's OpenMP C2R/R2C path (a port: the reference's
                 OpenMP transpose is declared but never implemented) on the same
                 two tensors, on this host's cores, bounded to ~10 s.
@@ -57,6 +62,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE child runs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--profile", action="store_true",
                     help="only run workload steps and the calibration copies (for profilers)")
     return ap.parse_args()
@@ -194,6 +200,46 @@ def measure_traffic(match, shape_esz):
     return totals, None
 
 
+def batch_leg(torch, ipt, count=4096, seed=4, reps=3):
+    """Per-call heuristic loop vs one planned, graph-captured batch."""
+    shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(count, 2, 4096, seed=seed)]
+    offs, total = [], 0
+    for d1, d2, d3, esz in shapes:
+        offs.append(total)
+        total += (d1 * d2 * d3 * esz + 255) // 256 * 256
+    buf = torch.empty(total // 8 + 1, dtype=torch.int64, device="cuda")
+    raw = buf.view(torch.uint8)
+    items = []
+    for (d1, d2, d3, esz), off in zip(shapes, offs):
+        n = d1 * d2 * d3
+        items.append((raw[off:off + n * esz].view(torch.int32 if esz == 4 else torch.int64),
+                      d1, d2, d3))
+    payload = sum(d1 * d2 * d3 * esz for d1, d2, d3, esz in shapes)
+    t0 = time.perf_counter()
+    batch = ipt.Batch(items)
+    plan_s = time.perf_counter() - t0
+    batch.run(graph=True)  # capture + first replay
+    torch.cuda.synchronize()
+
+    def per_call():
+        for t, d1, d2, d3 in items:
+            ipt.transpose(t, d1, d2, d3)
+
+    ms_loop = time_calls(torch, lambda _: per_call(), None, reps)
+    ms_batch = time_calls(torch, lambda _: batch.run(graph=True), None, reps)
+    batch.check()
+    info = batch.info()
+    batch.close()
+    del items, raw, buf
+    torch.cuda.empty_cache()
+    return {"matrices": count, "payload_bytes": payload, "plan_ms": round(plan_s * 1e3, 1),
+            "ms_per_call_loop": round(ms_loop, 3), "ms_batch_graph": round(ms_batch, 3),
+            "speedup": round(ms_loop / ms_batch, 2),
+            "batch_GBps": round(2 * payload / (ms_batch * 1e-3) / 1e9, 1),
+            "items": {k: info[k] for k in ("n_slice_items", "n_gcs_items", "n_swap_items",
+                                            "n_big", "grid")}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -277,6 +323,8 @@ def main():
                 "traffic_note": traffic_note, "alg_bytes": dom["alg_bytes"],
                 "kernel": dom["kernel"], "shape": dom["shape"]}
 
+    batched = None if args.no_batch else batch_leg(torch, ipt)
+
     cpu = None
     if not args.no_cpu:
         import oracle_ctypes as oracle
@@ -304,6 +352,7 @@ def main():
         "wall_s": round(wall, 3),
         "roofline": roofline,
         "kernels": kernels,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -149,6 +149,45 @@ ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, 
  * calibration reference for bandwidth and for the HBM traffic counters. */
 ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream);
 
+/* ---- batches of uneven transposes ------------------------------------- */
+/* No counterpart in the reference (one tensor per call, src/transpose.cu:
+ * 99-124).  A batch is planned once: every instance becomes work items of one
+ * persistent kernel (whole-slice LDS, cooperative group, or square tile-pair
+ * items); instances too large for those run through the per-call C2R / R2C
+ * launchers (only their LDS tiers: a batch never uses library scratch).
+ * ipt_batch_run captures everything once into a hipGraph on the first call
+ * and replays it afterwards.  Instance buffers must stay valid and must not
+ * overlap; a batch must not run concurrently with itself. */
+typedef struct {
+    void* data;          /* device buffer of d1*d2*d3 elements */
+    int32_t elem_bytes;  /* 4 or 8 */
+    int32_t variant;     /* IPT_VARIANT_AUTO or a forced variant */
+    int64_t d1, d2, d3;  /* as ipt_transpose_213 */
+} ipt_matrix_desc;
+
+typedef struct {
+    int64_t count, skipped, n_instances, n_big;
+    int64_t n_slice_items, n_gcs_items, n_swap_items;
+    int64_t grid, lds_bytes;
+    uint64_t bytes;      /* payload bytes of the transposed instances */
+} ipt_batch_info;
+
+typedef struct ipt_batch_s ipt_batch;
+
+/* tune may be NULL; tune->lds_bytes caps the persistent kernel's LDS. */
+ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const ipt_tune* tune,
+                            ipt_batch** out);
+/* One graph launch on `stream` (captured on the first call). */
+ipt_status ipt_batch_run(ipt_batch* b, void* stream);
+/* The same work launched directly, without a graph. */
+ipt_status ipt_batch_run_eager(ipt_batch* b, void* stream);
+ipt_status ipt_batch_get_info(const ipt_batch* b, ipt_batch_info* info);
+/* Synchronises the device and reports (IPT_ERR_HIP) whether a group barrier
+ * of this batch gave up waiting; the barrier state is then reset. */
+ipt_status ipt_batch_check(ipt_batch* b);
+/* Synchronises the device, then frees the batch. */
+ipt_status ipt_batch_destroy(ipt_batch* b);
+
 /* Synchronises the current device and reports whether any group barrier of
  * the cooperative kernels gave up waiting (IPT_ERR_HIP, with detail in
  * ipt_last_error; the barrier state is then reset).  IPT_OK otherwise. */

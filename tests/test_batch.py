@@ -1,0 +1,121 @@
+"""Batched uneven transposes: one persistent launch (+ graph capture) against
+the oracle, per instance, bit-exact."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+
+def carve(total_elems_bytes):
+    return torch.empty(total_elems_bytes // 8 + 64, dtype=torch.int64, device="cuda")
+
+
+def make_instances(oracle, shapes, seed):
+    """One device buffer carved into 256-byte aligned instances."""
+    offs, nbytes = [], 0
+    for d1, d2, d3, esz in shapes:
+        offs.append(nbytes)
+        nbytes += (d1 * d2 * d3 * esz + 255) // 256 * 256
+    buf = torch.empty(nbytes // 8 + 1, dtype=torch.int64, device="cuda")
+    raw = buf.view(torch.uint8)
+    items, host = [], []
+    for k, ((d1, d2, d3, esz), off) in enumerate(zip(shapes, offs)):
+        n = d1 * d2 * d3
+        a = oracle.random_bits(n, esz, seed + k)
+        t = raw[off:off + n * esz].view(torch.int32 if esz == 4 else torch.int64)
+        t.copy_(torch.from_numpy(a.view(np.int32 if esz == 4 else np.int64)))
+        items.append((t, d1, d2, d3))
+        host.append(a)
+    return buf, items, host
+
+
+def check(oracle, items, host, times):
+    for (t, d1, d2, d3), a in zip(items, host):
+        exp = a
+        for _ in range(times):
+            exp = oracle.seq_tt_213(exp, d1, d2, d3)
+        got = t.cpu().numpy().view(a.dtype)
+        assert np.array_equal(got, exp), (d1, d2, d3, a.itemsize)
+
+
+SHAPES = [  # (d1, d2, d3, esz): slice-in-LDS, GCS, square-swap and big routes, plus no-ops
+    (1, 1, 1, 4), (7, 1, 3, 8), (3, 5, 7, 4), (33, 17, 4, 8), (100, 30, 2, 4), (64, 64, 3, 8),
+    (512, 384, 2, 4), (256, 1000, 1, 8), (1000, 999, 1, 4), (3000, 1000, 1, 4), (97, 203, 5, 8),
+    (1024, 1024, 1, 4), (2048, 2048, 1, 8), (4096, 40, 1, 4), (40, 4096, 1, 8), (129, 131, 1, 4),
+    (2, 3000, 2, 8), (3000, 2, 1, 4), (700, 700, 2, 4), (255, 257, 1, 8), (4096, 4096, 1, 4),
+]
+
+
+@pytest.mark.gpu
+def test_batch_eager_and_graph_replays(ipt, oracle):
+    buf, items, host = make_instances(oracle, SHAPES, 900)
+    b = ipt.Batch(items)
+    info = b.info()
+    print("[batch]", info, flush=True)
+    assert info["n_slice_items"] > 0 and info["n_gcs_items"] > 0 and info["n_swap_items"] > 0
+    assert info["skipped"] == 2
+    b.run(graph=False)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, items, host, 1)
+    # graph: captured on first run, replayed; eager work and syncs in between
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    check(oracle, items, host, 2)
+    scratch = torch.zeros(1 << 20, device="cuda")
+    scratch += 1
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    check(oracle, items, host, 3)
+    b.run(graph=True)
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, items, host, 5)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batch_forced_variants_and_small_lds(ipt, oracle):
+    shapes = [(300, 100, 2, 4), (100, 300, 1, 8), (512, 512, 1, 4), (768, 512, 1, 4)]
+    variants = [ipt.VARIANT_GCS, ipt.VARIANT_R2C, ipt.VARIANT_SQUARE_SWAP, ipt.VARIANT_C2R]
+    buf, items, host = make_instances(oracle, shapes, 950)
+    items = [it + (v,) for it, v in zip(items, variants)]
+    b = ipt.Batch(items, tune=ipt.Tune(lds_bytes=65536))
+    info = b.info()
+    assert info["n_big"] == 2 and info["n_gcs_items"] > 0 and info["n_swap_items"] > 0
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, [it[:4] for it in items], host, 1)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_uneven_workload_sample(ipt, oracle):
+    """A 300-instance sample of the uneven-batch workload (m, n log-uniform in
+    [2, 1024]), one graph launch, every instance checked."""
+    shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(300, 2, 1024, seed=3)]
+    buf, items, host = make_instances(oracle, shapes, 1000)
+    b = ipt.Batch(items)
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, items, host, 1)
+    b.close()
+
+
+def test_batch_rejects_bad_descriptors(ipt):
+    import ctypes
+    L = ipt.lib()
+    descs = (ipt.MatrixDesc * 1)()
+    descs[0] = ipt.MatrixDesc(None, 4, 0, 4, 4, 1)
+    h = ctypes.c_void_p()
+    assert L.ipt_batch_create(descs, 1, None, ctypes.byref(h)) == 1
+    descs[0] = ipt.MatrixDesc(256, 3, 0, 4, 4, 1)
+    assert L.ipt_batch_create(descs, 1, None, ctypes.byref(h)) == 1
+    descs[0] = ipt.MatrixDesc(256, 4, 9, 4, 4, 1)
+    assert L.ipt_batch_create(descs, 1, None, ctypes.byref(h)) == 1
+    assert L.ipt_batch_create(descs, -1, None, ctypes.byref(h)) == 1
+    assert L.ipt_batch_run(None, None) == 1
+    assert L.ipt_batch_destroy(None) == 1
