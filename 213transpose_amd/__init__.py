@@ -79,6 +79,14 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    # torch wheels bundle their own HIP runtime under the same soname as the
+    # system one (libamdhip64.so.7); whichever is loaded first serves the
+    # whole process.  Load torch's first so a process that uses both has one
+    # runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise TransposeError(-1, "libinplace_hip.so not built",
                              f"expected {LIB_PATH}; run __graft_entry__.build()")
@@ -120,6 +128,27 @@ def lib():
     L.ipt_batch_check.argtypes = [_vp]
     L.ipt_batch_destroy.restype = _int
     L.ipt_batch_destroy.argtypes = [_vp]
+    L.ipt_candidate.restype = _int
+    L.ipt_candidate.argtypes = [_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Tune)]
+    L.ipt_shape_features.restype = _int
+    L.ipt_shape_features.argtypes = [_int, _i64, _i64, _i64, _vp]
+    for name in ("ipt_heuristic_candidate", "ipt_select_candidate"):
+        getattr(L, name).restype = _int
+        getattr(L, name).argtypes = [_int, _i64, _i64, _i64]
+    L.ipt_autotune.restype = _int
+    L.ipt_autotune.argtypes = [_vp, _int, _i64, _i64, _i64, _int, _vp, ctypes.POINTER(ctypes.c_int)]
+    L.ipt_selector_train.restype = _int
+    L.ipt_selector_train.argtypes = [_vp, _vp, _vp, _i64, _int, ctypes.c_float, _vp, _vp]
+    L.ipt_selector_grad.restype = _int
+    L.ipt_selector_grad.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp]
+    L.ipt_selector_infer.restype = _int
+    L.ipt_selector_infer.argtypes = [_vp, _vp, _i64, _vp]
+    L.ipt_selector_infer_host.restype = _int
+    L.ipt_selector_infer_host.argtypes = [_vp, _vp, _i64, _vp]
+    L.ipt_selector_active.restype = _int
+    L.ipt_selector_active.argtypes = []
+    L.ipt_selector_load.restype = _int
+    L.ipt_selector_load.argtypes = [_vp]
     L.ipt_version.restype = ctypes.c_char_p
     L.ipt_version.argtypes = []
     _lib = L
@@ -290,6 +319,139 @@ def uneven_shapes(count, lo=2, hi=4096, seed=0):
     mn = np.minimum(np.floor(np.exp(lm)).astype(np.int64), hi)
     esz = np.where(rng.random(count) < 0.5, 4, 8)
     return [(int(m), int(n), int(e)) for (m, n), e in zip(mn, esz)]
+
+
+# ---------------------------------------------------------------- selector
+
+N_CANDIDATES = 8
+MLP_IN, MLP_HIDDEN = 8, 64
+MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + \
+    N_CANDIDATES * MLP_HIDDEN + N_CANDIDATES
+SELECTOR_WEIGHTS = os.path.join(_HERE, "selector", "weights.f32")
+
+
+def candidate(k):
+    """(variant, Tune) of candidate k."""
+    v, t = ctypes.c_int(), Tune()
+    _check(lib().ipt_candidate(k, ctypes.byref(v), ctypes.byref(t)), "ipt_candidate")
+    return v.value, t
+
+
+def shape_features(esz, d1, d2, d3=1):
+    import numpy as np
+    out = np.empty(MLP_IN, dtype=np.float32)
+    _check(lib().ipt_shape_features(esz, d1, d2, d3, out.ctypes.data), "ipt_shape_features")
+    return out
+
+
+def heuristic_candidate(esz, d1, d2, d3=1):
+    return lib().ipt_heuristic_candidate(esz, d1, d2, d3)
+
+
+def select_candidate(esz, d1, d2, d3=1):
+    return lib().ipt_select_candidate(esz, d1, d2, d3)
+
+
+def autotune(data, d1, d2, d3, reps=3):
+    """Time every candidate on `data` (permuted in the process): (times_us, best)."""
+    import numpy as np
+    ptr, esz = _args(data, d1, d2, d3)
+    times = np.empty(N_CANDIDATES, dtype=np.float32)
+    best = ctypes.c_int()
+    _check(lib().ipt_autotune(ptr, esz, d1, d2, d3, reps, times.ctypes.data, ctypes.byref(best)),
+           "ipt_autotune")
+    return times, best.value
+
+
+def _f32(a, shape):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise TransposeError(1, f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def selector_train(x, y, mask, params, epochs=2000, lr=3e-3):
+    """Train on the GPU (the library's HIP kernels); returns (params, final loss)."""
+    n = len(x)
+    x = _f32(x, (n, MLP_IN))
+    y = _f32(y, (n, N_CANDIDATES))
+    mask = _f32(mask, (n, N_CANDIDATES))
+    p = _f32(params, (MLP_PARAMS,)).copy()
+    loss = ctypes.c_float()
+    _check(lib().ipt_selector_train(x.ctypes.data, y.ctypes.data, mask.ctypes.data, n, epochs,
+                                    lr, p.ctypes.data, ctypes.byref(loss)), "ipt_selector_train")
+    return p, loss.value
+
+
+def selector_grad(x, y, mask, params):
+    """Masked-MSE loss and its gradient, computed by the GPU kernels."""
+    import numpy as np
+    n = len(x)
+    x = _f32(x, (n, MLP_IN))
+    y = _f32(y, (n, N_CANDIDATES))
+    mask = _f32(mask, (n, N_CANDIDATES))
+    p = _f32(params, (MLP_PARAMS,))
+    g = np.empty(MLP_PARAMS, dtype=np.float32)
+    loss = ctypes.c_float()
+    _check(lib().ipt_selector_grad(x.ctypes.data, y.ctypes.data, mask.ctypes.data, n,
+                                   p.ctypes.data, ctypes.byref(loss), g.ctypes.data),
+           "ipt_selector_grad")
+    return loss.value, g
+
+
+def selector_infer(params, x):
+    """GPU inference kernel: predicted log2 microseconds per candidate."""
+    import numpy as np
+    n = len(x)
+    x = _f32(x, (n, MLP_IN))
+    p = _f32(params, (MLP_PARAMS,))
+    y = np.empty((n, N_CANDIDATES), dtype=np.float32)
+    _check(lib().ipt_selector_infer(p.ctypes.data, x.ctypes.data, n, y.ctypes.data),
+           "ipt_selector_infer")
+    return y
+
+
+def selector_infer_host(params, x):
+    """Host inference (bit-identical to selector_infer)."""
+    import numpy as np
+    n = len(x)
+    x = _f32(x, (n, MLP_IN))
+    p = _f32(params, (MLP_PARAMS,))
+    y = np.empty((n, N_CANDIDATES), dtype=np.float32)
+    _check(lib().ipt_selector_infer_host(p.ctypes.data, x.ctypes.data, n, y.ctypes.data),
+           "ipt_selector_infer_host")
+    return y
+
+
+def selector_load(params):
+    """Install MLP parameters for dispatch (None restores the heuristic)."""
+    if params is None:
+        _check(lib().ipt_selector_load(None), "ipt_selector_load")
+        return
+    p = _f32(params, (MLP_PARAMS,))
+    _check(lib().ipt_selector_load(p.ctypes.data), "ipt_selector_load")
+
+
+def selector_active():
+    """True when the learned selector decides VARIANT_AUTO (the library loads
+    the shipped weights by default)."""
+    return bool(lib().ipt_selector_active())
+
+
+def load_default_selector():
+    """Install the trained weights shipped with the package (if present).
+    Returns True when installed; a missing or corrupt file leaves the
+    heuristic in place."""
+    import numpy as np
+    try:
+        p = np.fromfile(SELECTOR_WEIGHTS, dtype=np.float32)
+    except OSError:
+        return False
+    if p.size != MLP_PARAMS or not np.isfinite(p).all():
+        return False
+    selector_load(p)
+    return True
 
 
 def device_check():

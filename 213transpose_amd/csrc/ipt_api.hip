@@ -7,6 +7,7 @@
 // device) and reports every failure as an ipt_status.
 #include "../../include/inplace_transpose.h"
 #include "ipt_internal.h"
+#include "ipt_select.h"
 
 #include <hip/hip_runtime.h>
 
@@ -71,7 +72,7 @@ ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr) {
         if (a.ptr) (void)hipFree(a.ptr);
         a.ptr = nullptr;
         a.bytes = 0;
-        size_t want = a.zero_at_rest ? std::max(bytes, (size_t)1 << 16)
+        size_t want = a.zero_at_rest ? std::max(bytes, (size_t)1 << 20)
                                      : std::max(bytes, (size_t)64 << 20);
         e = hipMalloc(&a.ptr, want);
         if (e != hipSuccess) {
@@ -171,6 +172,10 @@ ipt_status ipt_transpose_variant(void* data, int elem_bytes, int64_t d1, int64_t
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
     Shape s = make_shape(elem_bytes, d1, d2, d3);
+    if (variant == IPT_VARIANT_AUTO && !tune && selector_loaded()) {
+        const Candidate& c = kCandidates[select_candidate(ctx, s)];
+        return run_variant(ctx, data, s, c.variant, resolve_tune(ctx, &c.tune), (hipStream_t)stream);
+    }
     Tune t = resolve_tune(ctx, tune);
     if (variant == IPT_VARIANT_AUTO) variant = choose_variant(ctx, s);
     return run_variant(ctx, data, s, variant, t, (hipStream_t)stream);

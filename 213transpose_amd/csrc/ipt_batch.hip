@@ -7,6 +7,7 @@
 // launch geometry), and every run is one graph launch.
 #include "ipt_internal.h"
 #include "ipt_kernels.h"
+#include "ipt_select.h"
 
 #include <algorithm>
 #include <cstring>
@@ -144,6 +145,14 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     }
     const int grid = per_cu * ncu_of(ctx);
 
+    // Learned choices for every AUTO instance, scored on the GPU in one call.
+    std::vector<Shape> auto_shapes;
+    for (int64_t i = 0; i < count; i++)
+        if (descs[i].variant == IPT_VARIANT_AUTO)
+            auto_shapes.push_back(make_shape(descs[i].elem_bytes, descs[i].d1, descs[i].d2, descs[i].d3));
+    const std::vector<int> learned = select_candidates_gpu(ctx, auto_shapes);
+    size_t next_auto = 0;
+
     std::vector<BatchInst> insts;
     std::vector<BatchItem> gcs_items, swap_items, slice_items;
     std::vector<std::pair<uint64_t, BatchItem>> slice_sorted;
@@ -153,7 +162,10 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         const ipt_matrix_desc& d = descs[i];
         const Shape s = make_shape(d.elem_bytes, d.d1, d.d2, d.d3);
         int big_variant = IPT_VARIANT_AUTO;
-        const Route p = plan_instance(ctx, s, d.variant, lds_cap, grid, b->tune, &big_variant);
+        int forced = d.variant;
+        if (forced == IPT_VARIANT_AUTO && !learned.empty())
+            forced = kCandidates[learned[next_auto++]].variant;
+        const Route p = plan_instance(ctx, s, forced, lds_cap, grid, b->tune, &big_variant);
         if (p == Route::Skip) {
             b->info.skipped++;
             continue;

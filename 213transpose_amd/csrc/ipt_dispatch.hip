@@ -228,13 +228,20 @@ static ipt_status launch_gcs(DeviceCtx* ctx, T* data, const Shape& s, const GcsG
     // group must be running while the others wait for it.
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(slots, resident));
     (void)t;
+    if ((uint64_t)grid < gg.g) return IPT_ERR_UNSUPPORTED;  // a group must be co-resident
+    if (gg.g == 1) {  // whole slices per workgroup: no barrier state needed
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, data, (uint32_t)s.d2,
+                           (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL, Div32::make(gg.ntj),
+                           gg.ntiles, (uint32_t*)nullptr, (uint32_t*)nullptr);
+        IPT_LAUNCH_CHECK("k_gcs");
+        return IPT_OK;
+    }
     std::lock_guard<std::mutex> lock(ctx->mu);
     void* sync;
     stt = arena_acquire(ctx->sync, (size_t)(32 + bar_words(gg.g) * (uint64_t)s.d3) * sizeof(uint32_t),
                         st, &sync);
     if (stt != IPT_OK) return stt;
     uint32_t* ctl = (uint32_t*)sync;
-    if ((uint64_t)grid < gg.g) return IPT_ERR_UNSUPPORTED;  // a group must be co-resident
     hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, data, (uint32_t)s.d2,
                        (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL, Div32::make(gg.ntj),
                        gg.ntiles, ctl, ctl + 32);

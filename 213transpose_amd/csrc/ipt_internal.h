@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <vector>
 
 #include "../../include/inplace_transpose.h"
 #include "ipt_plan.h"
@@ -64,6 +65,12 @@ Shape make_shape(int esz, int64_t d1, int64_t d2, int64_t d3);
 Tune resolve_tune(const DeviceCtx* ctx, const ipt_tune* t);
 bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant);
 int choose_variant(const DeviceCtx* ctx, const Shape& s);
+// Candidate (index into the selector's candidate list) the dispatcher runs for
+// IPT_VARIANT_AUTO: the learned selector's choice when one is loaded, else the
+// heuristic's.
+int select_candidate(const DeviceCtx* ctx, const Shape& s);
+bool selector_loaded();
+std::vector<int> select_candidates_gpu(const DeviceCtx* ctx, const std::vector<Shape>& shapes);
 ipt_status run_variant(DeviceCtx* ctx, void* data, const Shape& s, int variant, const Tune& t,
                        hipStream_t stream);
 ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st);

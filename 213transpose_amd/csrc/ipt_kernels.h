@@ -476,7 +476,7 @@ k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
         gcs_body<T, TS, VEC, BLOCK, RV>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
-                                        cnt + bar_words(g) * S, ctl, lds);
+                                        g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds);
         __syncthreads();  // LDS is reused by the next slot
     }
 }

@@ -201,7 +201,8 @@ def measure_traffic(match, shape_esz):
 
 
 def batch_leg(torch, ipt, count=4096, seed=4, reps=3):
-    """Per-call heuristic loop vs one planned, graph-captured batch."""
+    """Plain per-call heuristic dispatch vs the learned selector + one planned,
+    graph-captured batch, on the uneven-batch workload."""
     shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(count, 2, 4096, seed=seed)]
     offs, total = [], 0
     for d1, d2, d3, esz in shapes:
@@ -215,29 +216,55 @@ def batch_leg(torch, ipt, count=4096, seed=4, reps=3):
         items.append((raw[off:off + n * esz].view(torch.int32 if esz == 4 else torch.int64),
                       d1, d2, d3))
     payload = sum(d1 * d2 * d3 * esz for d1, d2, d3, esz in shapes)
-    t0 = time.perf_counter()
-    batch = ipt.Batch(items)
-    plan_s = time.perf_counter() - t0
-    batch.run(graph=True)  # capture + first replay
-    torch.cuda.synchronize()
 
     def per_call():
         for t, d1, d2, d3 in items:
             ipt.transpose(t, d1, d2, d3)
 
-    ms_loop = time_calls(torch, lambda _: per_call(), None, reps)
-    ms_batch = time_calls(torch, lambda _: batch.run(graph=True), None, reps)
-    batch.check()
-    info = batch.info()
-    batch.close()
+    def batched():
+        t0 = time.perf_counter()
+        b = ipt.Batch(items)
+        plan_ms = (time.perf_counter() - t0) * 1e3
+        b.run(graph=True)  # capture + first replay
+        torch.cuda.synchronize()
+        ms = time_calls(torch, lambda _: b.run(graph=True), None, reps)
+        b.check()
+        info = b.info()
+        b.close()
+        return ms, plan_ms, info
+
+    ipt.selector_load(None)  # plain heuristic
+    ms_loop_h = time_calls(torch, lambda _: per_call(), None, reps)
+    ms_batch_h, _, _ = batched()
+    learned = ipt.load_default_selector()
+    ms_loop_l = time_calls(torch, lambda _: per_call(), None, reps)
+    ms_batch_l, plan_ms, info = batched()
     del items, raw, buf
     torch.cuda.empty_cache()
-    return {"matrices": count, "payload_bytes": payload, "plan_ms": round(plan_s * 1e3, 1),
-            "ms_per_call_loop": round(ms_loop, 3), "ms_batch_graph": round(ms_batch, 3),
-            "speedup": round(ms_loop / ms_batch, 2),
-            "batch_GBps": round(2 * payload / (ms_batch * 1e-3) / 1e9, 1),
+    return {"matrices": count, "payload_bytes": payload, "selector_loaded": learned,
+            "ms_per_call_heuristic": round(ms_loop_h, 3),
+            "ms_per_call_learned": round(ms_loop_l, 3),
+            "ms_batch_heuristic": round(ms_batch_h, 3),
+            "ms_batch_learned": round(ms_batch_l, 3),
+            "speedup": round(ms_loop_h / ms_batch_l, 2),
+            "speedup_note": "t(per-call heuristic) / t(learned + batched graph)",
+            "batch_GBps": round(2 * payload / (ms_batch_l * 1e-3) / 1e9, 1),
+            "plan_ms": round(plan_ms, 1),
             "items": {k: info[k] for k in ("n_slice_items", "n_gcs_items", "n_swap_items",
                                             "n_big", "grid")}}
+
+
+def selector_eval(ipt):
+    """Held-out evaluation of the shipped selector on its own sweep data (CPU)."""
+    path = os.path.join(ROOT, "213transpose_amd", "selector", "sweep.npz")
+    if not os.path.exists(path) or not os.path.exists(ipt.SELECTOR_WEIGHTS):
+        return None
+    import numpy as np
+    at = importlib.import_module("213transpose_amd.autotune")
+    d = np.load(path)
+    p = np.fromfile(ipt.SELECTOR_WEIGHTS, dtype=np.float32)
+    res = at.evaluate_arrays(d["shapes"][-2000:], d["feats"][-2000:], d["times"][-2000:], p)
+    return {k: round(v, 4) for k, v in res.items()}
 
 
 def main():
@@ -324,6 +351,7 @@ def main():
                 "kernel": dom["kernel"], "shape": dom["shape"]}
 
     batched = None if args.no_batch else batch_leg(torch, ipt)
+    sel_eval = selector_eval(ipt)
 
     cpu = None
     if not args.no_cpu:
@@ -353,6 +381,8 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "batched": batched,
+        "selector_heldout": sel_eval,
+        "dispatch": "learned selector" if ipt.selector_active() else "heuristic",
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

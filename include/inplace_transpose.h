@@ -149,6 +149,68 @@ ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, 
  * calibration reference for bandwidth and for the HBM traffic counters. */
 ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream);
 
+/* ---- autotuner and learned selector ----------------------------------- */
+/* No counterpart in the reference: its tier thresholds are magic numbers
+ * (src/shuffle.cu:15-50, src/rotate.cu:241-258).  Here IPT_NUM_CANDIDATES
+ * fixed configurations (variant + knobs, listed by ipt_candidate) are timed by
+ * the autotuner; a small MLP (IPT_MLP_IN -> IPT_MLP_HIDDEN -> IPT_MLP_HIDDEN ->
+ * IPT_NUM_CANDIDATES, fp32) trained on those timings predicts the log2 time
+ * (microseconds) of each candidate, and the dispatcher runs the feasible
+ * candidate with the lowest prediction, for problems up to 1 GiB of payload
+ * (the domain of its training sweep).  Without a loaded selector, and above
+ * that size, the plain heuristic decides. */
+#define IPT_NUM_CANDIDATES 8
+#define IPT_MLP_IN 8
+#define IPT_MLP_HIDDEN 64
+/* Number of fp32 parameters: W1, b1, W2, b2, W3, b3 (row-major, out x in). */
+#define IPT_MLP_PARAMS (IPT_MLP_HIDDEN * IPT_MLP_IN + IPT_MLP_HIDDEN + \
+                        IPT_MLP_HIDDEN * IPT_MLP_HIDDEN + IPT_MLP_HIDDEN + \
+                        IPT_NUM_CANDIDATES * IPT_MLP_HIDDEN + IPT_NUM_CANDIDATES)
+
+/* Candidate k: its variant and knobs. */
+ipt_status ipt_candidate(int k, int* variant, ipt_tune* tune);
+/* Features of a shape (IPT_MLP_IN floats; host only). */
+ipt_status ipt_shape_features(int elem_bytes, int64_t d1, int64_t d2, int64_t d3, float* out);
+/* The candidate the plain heuristic picks for a shape (host only). */
+int ipt_heuristic_candidate(int elem_bytes, int64_t d1, int64_t d2, int64_t d3);
+
+/* Autotuner: time every candidate on `data` (contents are permuted; any
+ * values will do), one warm-up run then `reps` timed runs each, HIP events
+ * on an internal stream.  times_us[k] is the mean time in microseconds, or a
+ * negative value when candidate k is infeasible for the shape.  *best is the
+ * fastest candidate. */
+ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3, int reps,
+                        float* times_us, int* best);
+
+/* Train the MLP on the GPU with the library's HIP kernels (full-batch Adam on
+ * masked squared error of log2 times).  x: n x IPT_MLP_IN features; y: n x
+ * IPT_NUM_CANDIDATES log2 times; mask: n x IPT_NUM_CANDIDATES, 1 where y is
+ * valid.  params: IPT_MLP_PARAMS floats, initial values in, trained values out.
+ * loss_out (may be NULL): final masked MSE. */
+ipt_status ipt_selector_train(const float* x, const float* y, const float* mask, int64_t n,
+                              int epochs, float lr, float* params, float* loss_out);
+/* One forward + backward pass on the GPU (for gradient checks): the masked
+ * MSE loss and its gradient with respect to every parameter. */
+ipt_status ipt_selector_grad(const float* x, const float* y, const float* mask, int64_t n,
+                             const float* params, float* loss_out, float* grad_out);
+/* GPU inference kernel: y = MLP(x) for n rows (n x IPT_NUM_CANDIDATES out). */
+ipt_status ipt_selector_infer(const float* params, const float* x, int64_t n, float* y);
+/* The same inference on the host (bit-identical to the GPU kernel: both
+ * accumulate with fused multiply-adds in the same order). */
+ipt_status ipt_selector_infer_host(const float* params, const float* x, int64_t n, float* y);
+/* Install parameters for dispatch (copied; NULL uninstalls and restores the
+ * heuristic).  Affects IPT_VARIANT_AUTO in every entry point and batches
+ * created afterwards.  Until this is called, the library uses the trained
+ * weights shipped with it (selector/weights.f32 next to the shared object,
+ * or the file named by $IPT_SELECTOR_WEIGHTS); IPT_SELECTOR=heuristic
+ * disables them, and a missing or corrupt file leaves the heuristic. */
+ipt_status ipt_selector_load(const float* params);
+/* 1 when a selector decides IPT_VARIANT_AUTO, 0 when the heuristic does. */
+int ipt_selector_active(void);
+/* The candidate the dispatcher would run for a shape now (selector if
+ * loaded, else heuristic); host only. */
+int ipt_select_candidate(int elem_bytes, int64_t d1, int64_t d2, int64_t d3);
+
 /* ---- batches of uneven transposes ------------------------------------- */
 /* No counterpart in the reference (one tensor per call, src/transpose.cu:
  * 99-124).  A batch is planned once: every instance becomes work items of one

@@ -1,0 +1,180 @@
+"""Learned selector: MLP math, GPU training kernels, gradients, dispatch.
+
+The float64 numpy MLP below is the reference for the library's fp32 host
+inference and for the gradients of its HIP backward kernels; the numpy
+backward is itself pinned by central finite differences.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DATA = os.path.join(ROOT, "213transpose_amd", "selector", "sweep.npz")
+WEIGHTS = os.path.join(ROOT, "213transpose_amd", "selector", "weights.f32")
+HELDOUT = 2000
+
+
+def unpack(p, nin=8, nh=64, nout=8):
+    i = 0
+    out = []
+    for a, b in ((nh, nin), (nh, 1), (nh, nh), (nh, 1), (nout, nh), (nout, 1)):
+        out.append(p[i:i + a * b].reshape(a, b) if b > 1 else p[i:i + a])
+        i += a * b
+    return out
+
+
+def forward64(p, x):
+    W1, b1, W2, b2, W3, b3 = unpack(np.asarray(p, np.float64))
+    z1 = x @ W1.T + b1
+    h1 = np.maximum(z1, 0)
+    z2 = h1 @ W2.T + b2
+    h2 = np.maximum(z2, 0)
+    return h1, h2, h2 @ W3.T + b3
+
+
+def loss_grad64(p, x, y, mask):
+    x = np.asarray(x, np.float64)
+    W1, b1, W2, b2, W3, b3 = unpack(np.asarray(p, np.float64))
+    h1, h2, yh = forward64(p, x)
+    s = 1.0 / mask.sum()
+    d = (yh - y) * mask
+    loss = s * (d ** 2).sum()
+    dy = 2 * s * d
+    gW3, gb3 = dy.T @ h2, dy.sum(0)
+    dh2 = (dy @ W3) * (h2 > 0)
+    gW2, gb2 = dh2.T @ h1, dh2.sum(0)
+    dh1 = (dh2 @ W2) * (h1 > 0)
+    gW1, gb1 = dh1.T @ x, dh1.sum(0)
+    return loss, np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+
+
+def problem(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, size=(n, 8)).astype(np.float32)
+    y = rng.normal(4, 2, size=(n, 8)).astype(np.float32)
+    mask = (rng.random((n, 8)) < 0.7).astype(np.float32)
+    p = (rng.normal(0, 0.3, size=8 * 64 + 64 + 64 * 64 + 64 + 8 * 64 + 8)).astype(np.float32)
+    return x, y, mask, p
+
+
+def test_numpy_backward_matches_finite_differences():
+    x, y, mask, p = problem(12, 1)
+    p = p.astype(np.float64)
+    _, g = loss_grad64(p, x, y, mask)
+    rng = np.random.default_rng(2)
+    for i in rng.choice(len(p), size=40, replace=False):
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        num = (loss_grad64(p + e, x, y, mask)[0] - loss_grad64(p - e, x, y, mask)[0]) / 2e-6
+        assert abs(num - g[i]) <= 1e-5 * max(1.0, abs(g[i])), (i, num, g[i])
+
+
+def test_host_inference_matches_float64(ipt):
+    x, _, _, p = problem(200, 3)
+    got = ipt.selector_infer_host(p, x)
+    exp = forward64(p, x.astype(np.float64))[2]
+    assert np.allclose(got, exp, rtol=1e-4, atol=1e-4)
+
+
+def test_shape_features(ipt):
+    f = ipt.shape_features(8, 3000, 1000, 4)
+    R, C = 1000, 3000
+    exp = [np.log2(R) / 16, np.log2(C) / 16, np.log2(1000) / 16, 1.0,
+           (np.log2(R) - np.log2(C)) / 16, np.log2(R * C * 8) / 32, 0.0, np.log2(4) / 16]
+    assert np.allclose(f, exp, atol=1e-6)
+    assert ipt.shape_features(4, 64, 64, 1)[6] == 1.0
+
+
+def test_candidates_and_heuristic_mapping(ipt):
+    seen = set()
+    for k in range(ipt.N_CANDIDATES):
+        v, _ = ipt.candidate(k)
+        seen.add(v)
+    assert seen == {ipt.VARIANT_SLICE_LDS, ipt.VARIANT_SQUARE_SWAP, ipt.VARIANT_GCS,
+                    ipt.VARIANT_C2R, ipt.VARIANT_R2C}
+    for shape in [(3000, 1000, 1), (512, 384, 64), (16384, 16384, 1), (7, 9, 1)]:
+        k = ipt.heuristic_candidate(4, *shape)
+        assert ipt.candidate(k)[0] == ipt.plan_query(4, *shape)["heuristic_variant"]
+
+
+def test_selector_load_rejects_non_finite(ipt):
+    p = np.zeros(ipt.MLP_PARAMS, np.float32)
+    p[5] = np.nan
+    with pytest.raises(ipt.TransposeError):
+        ipt.selector_load(p)
+    ipt.selector_load(None)
+
+
+@pytest.mark.skipif(not (os.path.exists(DATA) and os.path.exists(WEIGHTS)),
+                    reason="selector sweep / weights not committed")
+def test_trained_selector_beats_heuristic_on_heldout(ipt):
+    """Held-out shapes of the autotuner's own sweep: the learned choice's
+    measured time against exhaustive search and against the heuristic's."""
+    import importlib
+    at = importlib.import_module("213transpose_amd.autotune")
+    d = np.load(DATA)
+    p = np.fromfile(WEIGHTS, dtype=np.float32)
+    res = at.evaluate_arrays(d["shapes"][-HELDOUT:], d["feats"][-HELDOUT:], d["times"][-HELDOUT:], p)
+    print(json.dumps(res))
+    assert res["shapes"] >= 1900
+    assert res["total_time_speedup_vs_heuristic"] > 1.0
+    assert res["regret_mean"] < res["heuristic_regret_mean"]
+
+
+# ---------------------------------------------------------------- GPU
+
+@pytest.mark.gpu
+def test_gpu_gradients_match_float64(ipt):
+    x, y, mask, p = problem(300, 4)
+    loss, g = ipt.selector_grad(x, y, mask, p)
+    loss64, g64 = loss_grad64(p.astype(np.float64), x, y, mask)
+    assert abs(loss - loss64) <= 1e-4 * abs(loss64)
+    scale = np.abs(g64).max()
+    assert np.allclose(g, g64, rtol=1e-3, atol=1e-5 * scale), np.abs(g - g64).max()
+
+
+@pytest.mark.gpu
+def test_gpu_inference_bit_identical_to_host(ipt):
+    x, _, _, p = problem(1000, 5)
+    assert np.array_equal(ipt.selector_infer(p, x), ipt.selector_infer_host(p, x))
+
+
+@pytest.mark.gpu
+def test_gpu_training_fits_smooth_targets(ipt):
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0, 1, size=(2000, 8)).astype(np.float32)
+    w = rng.normal(size=(8, 8))
+    y = (np.sin(x @ w) * 3 + 5).astype(np.float32)
+    mask = np.ones_like(y)
+    p0 = np.concatenate([rng.normal(0, np.sqrt(2 / a), size=a * b) if k % 2 == 0 else np.zeros(b)
+                         for k, (a, b) in enumerate([(8, 64), (8, 64), (64, 64), (64, 64),
+                                                     (64, 8), (64, 8)])]).astype(np.float32)
+    loss0, _ = ipt.selector_grad(x, y, mask, p0)
+    p1, loss1 = ipt.selector_train(x, y, mask, p0, epochs=600, lr=3e-3)
+    assert loss1 < 0.05 * loss0, (loss0, loss1)
+    # the returned parameters reproduce the reported loss
+    assert abs(ipt.selector_grad(x, y, mask, p1)[0] - loss1) < 1e-4 * max(1.0, loss1)
+
+
+@pytest.mark.gpu
+def test_learned_dispatch_is_bit_exact(ipt, oracle):
+    import torch
+    rng = np.random.default_rng(7)
+    p = (rng.normal(0, 0.5, size=ipt.MLP_PARAMS)).astype(np.float32)
+    ipt.selector_load(p)
+    try:
+        used = set()
+        for k, (d1, d2, d3, esz) in enumerate([(300, 100, 2, 4), (512, 384, 8, 4), (256, 1000, 3, 8),
+                                               (1000, 1000, 1, 4), (33, 4096, 1, 8), (4096, 7, 2, 4),
+                                               (97, 203, 5, 8), (2000, 2001, 1, 4)]):
+            used.add(ipt.select_candidate(esz, d1, d2, d3))
+            a = oracle.random_bits(d1 * d2 * d3, esz, 1100 + k)
+            t = torch.from_numpy(a.view(np.int32 if esz == 4 else np.int64)).cuda()
+            ipt.transpose(t, d1, d2, d3)
+            torch.cuda.synchronize()
+            assert np.array_equal(t.cpu().numpy().view(a.dtype), oracle.seq_tt_213(a, d1, d2, d3))
+        print("[selector] candidates used", sorted(used))
+    finally:
+        ipt.selector_load(None)
