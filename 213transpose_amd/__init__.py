@@ -145,6 +145,11 @@ def lib():
     L.ipt_selector_infer.argtypes = [_vp, _vp, _i64, _vp]
     L.ipt_selector_infer_host.restype = _int
     L.ipt_selector_infer_host.argtypes = [_vp, _vp, _i64, _vp]
+    for name in ("ipt_tuning_save", "ipt_tuning_load"):
+        getattr(L, name).restype = _int
+        getattr(L, name).argtypes = [ctypes.c_char_p]
+    L.ipt_tuning_clear.restype = _int
+    L.ipt_tuning_clear.argtypes = []
     L.ipt_selector_active.restype = _int
     L.ipt_selector_active.argtypes = []
     L.ipt_selector_load.restype = _int
@@ -431,6 +436,18 @@ def selector_load(params):
         return
     p = _f32(params, (MLP_PARAMS,))
     _check(lib().ipt_selector_load(p.ctypes.data), "ipt_selector_load")
+
+
+def tuning_clear():
+    _check(lib().ipt_tuning_clear(), "ipt_tuning_clear")
+
+
+def tuning_save(path):
+    _check(lib().ipt_tuning_save(os.fsencode(path)), "ipt_tuning_save")
+
+
+def tuning_load(path):
+    _check(lib().ipt_tuning_load(os.fsencode(path)), "ipt_tuning_load")
 
 
 def selector_active():

@@ -48,6 +48,7 @@ struct ipt_batch_s {
         void* data;
         Shape s;
         int variant;
+        Tune tune;
     };
     std::vector<Big> big;  // instances run by the per-call launchers
     Tune tune{};
@@ -85,6 +86,29 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
     return Route::Big;
 }
 
+// Candidate for an AUTO instance of a batch: a measured choice if the
+// autotuner recorded one; else, with predictions y, the fastest predicted
+// route of the persistent kernel (whole slice, cooperative group, square
+// swap) when any fits, and only otherwise the fastest predicted C2R / R2C
+// candidate (separate launches cost far more inside a batch than their
+// standalone timings show).  -1 = let plan_instance's heuristic decide.
+int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t lds_cap, int grid) {
+    const int tuned = tuned_candidate(s);
+    if (tuned >= 0) return tuned;
+    if (!y || (uint64_t)s.d1 * s.d2 * s.d3 * s.esz > kSelectorMaxBytes) return -1;
+    const int64_t slice_bytes = s.d2 * slice_pitch(s.d1) * s.esz;
+    const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false);
+    const bool route_ok[4] = {slice_bytes <= lds_cap, s.d1 == s.d2,
+                              gg.ok && (int64_t)gg.g <= grid, gg.ok && (int64_t)gg.g <= grid};
+    int best = -1;
+    for (int k = 0; k < 4; k++)
+        if (route_ok[k] && (best < 0 || y[k] < y[best])) best = k;
+    if (best >= 0) return best;
+    for (int k = 4; k < kNumCandidates; k++)
+        if (candidate_ok(ctx, s, k) && (best < 0 || y[k] < y[best])) best = k;
+    return best;
+}
+
 ipt_status enqueue(ipt_batch* b, hipStream_t st) {
     if (b->nitems) {
         hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBatchBlock), b->lds, st, b->d_insts,
@@ -96,7 +120,7 @@ ipt_status enqueue(ipt_batch* b, hipStream_t st) {
         }
     }
     for (const auto& x : b->big) {
-        ipt_status r = run_variant(b->ctx, x.data, x.s, x.variant, b->tune, st);
+        ipt_status r = run_variant(b->ctx, x.data, x.s, x.variant, x.tune, st);
         if (r != IPT_OK) return r;
     }
     return IPT_OK;
@@ -145,12 +169,13 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     }
     const int grid = per_cu * ncu_of(ctx);
 
-    // Learned choices for every AUTO instance, scored on the GPU in one call.
+    // Predictions for every AUTO instance, scored on the GPU in one call.
     std::vector<Shape> auto_shapes;
     for (int64_t i = 0; i < count; i++)
         if (descs[i].variant == IPT_VARIANT_AUTO)
             auto_shapes.push_back(make_shape(descs[i].elem_bytes, descs[i].d1, descs[i].d2, descs[i].d3));
-    const std::vector<int> learned = select_candidates_gpu(ctx, auto_shapes);
+    std::vector<float> pred;
+    const bool have_pred = predict_gpu(auto_shapes, pred);
     size_t next_auto = 0;
 
     std::vector<BatchInst> insts;
@@ -163,8 +188,16 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         const Shape s = make_shape(d.elem_bytes, d.d1, d.d2, d.d3);
         int big_variant = IPT_VARIANT_AUTO;
         int forced = d.variant;
-        if (forced == IPT_VARIANT_AUTO && !learned.empty())
-            forced = kCandidates[learned[next_auto++]].variant;
+        Tune big_tune = b->tune;
+        if (forced == IPT_VARIANT_AUTO) {
+            const float* y = have_pred ? &pred[(next_auto++) * kNumCandidates] : nullptr;
+            const int k = batch_choice(ctx, s, y, lds_cap, grid);
+            if (k >= 0) {
+                forced = kCandidates[k].variant;
+                if (forced == IPT_VARIANT_C2R || forced == IPT_VARIANT_R2C)
+                    big_tune = resolve_tune(ctx, &kCandidates[k].tune);
+            }
+        }
         const Route p = plan_instance(ctx, s, forced, lds_cap, grid, b->tune, &big_variant);
         if (p == Route::Skip) {
             b->info.skipped++;
@@ -183,7 +216,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
                 release(b);
                 return IPT_ERR_UNSUPPORTED;
             }
-            b->big.push_back({d.data, s, big_variant});
+            b->big.push_back({d.data, s, big_variant, big_tune});
             b->info.n_big++;
             continue;
         }

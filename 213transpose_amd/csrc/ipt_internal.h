@@ -70,7 +70,9 @@ int choose_variant(const DeviceCtx* ctx, const Shape& s);
 // heuristic's.
 int select_candidate(const DeviceCtx* ctx, const Shape& s);
 bool selector_loaded();
-std::vector<int> select_candidates_gpu(const DeviceCtx* ctx, const std::vector<Shape>& shapes);
+bool predict_gpu(const std::vector<Shape>& shapes, std::vector<float>& y);
+int tuned_candidate(const Shape& s);  // -1 when the shape has no measured choice
+bool candidate_ok(const DeviceCtx* ctx, const Shape& s, int k);
 ipt_status run_variant(DeviceCtx* ctx, void* data, const Shape& s, int variant, const Tune& t,
                        hipStream_t stream);
 ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st);

@@ -102,6 +102,9 @@ static std::mutex g_sel_mu;
 static std::vector<float> g_sel_params;  // empty = no selector
 static std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> g_sel_cache;
 static bool g_sel_default_tried = false;
+// Measured choices: exact shape -> fastest candidate, filled by ipt_autotune
+// (and ipt_tuning_load).  Consulted before the selector.
+static std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> g_tuned;
 
 // The trained weights shipped with the library: $IPT_SELECTOR_WEIGHTS, else
 // selector/weights.f32 next to this shared object.  IPT_SELECTOR=heuristic
@@ -140,6 +143,8 @@ int select_candidate(const DeviceCtx* ctx, const Shape& s) {
     const bool in_domain = (uint64_t)s.d1 * s.d2 * s.d3 * s.esz <= kSelectorMaxBytes;
     {
         std::lock_guard<std::mutex> g(g_sel_mu);
+        auto tuned = g_tuned.find(std::make_tuple(s.esz, s.d1, s.d2, s.d3));
+        if (tuned != g_tuned.end() && candidate_feasible(ctx, s, tuned->second)) return tuned->second;
         load_default_weights_locked();
         if (!g_sel_params.empty() && in_domain) {
             auto key = std::make_tuple(s.esz, s.d1, s.d2, s.d3);
@@ -161,37 +166,32 @@ int select_candidate(const DeviceCtx* ctx, const Shape& s) {
     return heuristic_candidate(choose_variant(ctx, s));
 }
 
-// Selector choices for many shapes at once through the GPU inference kernel
-// (used when a batch is planned).  Empty when no selector is loaded.
-std::vector<int> select_candidates_gpu(const DeviceCtx* ctx, const std::vector<Shape>& shapes) {
+// Predicted log2 times (n x kOut) for many shapes at once through the GPU
+// inference kernel (used when a batch is planned).  False when no selector
+// is loaded or the shape set is empty.
+bool predict_gpu(const std::vector<Shape>& shapes, std::vector<float>& y) {
     std::vector<float> params;
     {
         std::lock_guard<std::mutex> g(g_sel_mu);
         load_default_weights_locked();
         params = g_sel_params;
     }
-    std::vector<int> out;
-    if (params.empty() || shapes.empty()) return out;
+    if (params.empty() || shapes.empty()) return false;
     const int64_t n = (int64_t)shapes.size();
-    std::vector<float> x((size_t)n * kFeat), y((size_t)n * kOut);
+    std::vector<float> x((size_t)n * kFeat);
+    y.assign((size_t)n * kOut, 0.0f);
     for (int64_t r = 0; r < n; r++)
         shape_features(shapes[r].esz, shapes[r].d1, shapes[r].d2, shapes[r].d3, &x[r * kFeat]);
-    if (ipt_selector_infer(params.data(), x.data(), n, y.data()) != IPT_OK) return out;
-    out.resize(n);
-    for (int64_t r = 0; r < n; r++) {
-        int best = -1;
-        const Shape& s = shapes[r];
-        if ((uint64_t)s.d1 * s.d2 * s.d3 * s.esz > kSelectorMaxBytes) {
-            out[r] = heuristic_candidate(choose_variant(ctx, s));
-            continue;
-        }
-        for (int k = 0; k < kOut; k++)
-            if (candidate_feasible(ctx, shapes[r], k) && (best < 0 || y[r * kOut + k] < y[r * kOut + best]))
-                best = k;
-        out[r] = best >= 0 ? best : heuristic_candidate(choose_variant(ctx, shapes[r]));
-    }
-    return out;
+    return ipt_selector_infer(params.data(), x.data(), n, y.data()) == IPT_OK;
 }
+
+int tuned_candidate(const Shape& s) {
+    std::lock_guard<std::mutex> g(g_sel_mu);
+    auto it = g_tuned.find(std::make_tuple(s.esz, s.d1, s.d2, s.d3));
+    return it == g_tuned.end() ? -1 : it->second;
+}
+
+bool candidate_ok(const DeviceCtx* ctx, const Shape& s, int k) { return candidate_feasible(ctx, s, k); }
 
 bool selector_loaded() {
     std::lock_guard<std::mutex> g(g_sel_mu);
@@ -231,6 +231,53 @@ int ipt_select_candidate(int elem_bytes, int64_t d1, int64_t d2, int64_t d3) {
     DeviceCtx* ctx = nullptr;
     if (get_ctx(&ctx) != IPT_OK) ctx = nullptr;
     return select_candidate(ctx, make_shape(elem_bytes, d1, d2, d3));
+}
+
+ipt_status ipt_tuning_clear(void) {
+    std::lock_guard<std::mutex> g(g_sel_mu);
+    g_tuned.clear();
+    g_sel_cache.clear();
+    return IPT_OK;
+}
+
+ipt_status ipt_tuning_save(const char* path) {
+    if (!path) return IPT_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> g(g_sel_mu);
+    FILE* f = std::fopen(path, "w");
+    if (!f) return IPT_ERR_IO;
+    std::fprintf(f, "# ipt tuning table: elem_bytes d1 d2 d3 candidate\n");
+    for (const auto& kv : g_tuned)
+        std::fprintf(f, "%d %lld %lld %lld %d\n", std::get<0>(kv.first),
+                     (long long)std::get<1>(kv.first), (long long)std::get<2>(kv.first),
+                     (long long)std::get<3>(kv.first), kv.second);
+    const bool ok = std::fclose(f) == 0;
+    return ok ? IPT_OK : IPT_ERR_IO;
+}
+
+ipt_status ipt_tuning_load(const char* path) {
+    if (!path) return IPT_ERR_INVALID_ARG;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return IPT_ERR_IO;
+    std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> table;
+    char line[256];
+    bool bad = false;
+    while (std::fgets(line, sizeof(line), f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int esz, k;
+        long long d1, d2, d3;
+        if (std::sscanf(line, "%d %lld %lld %lld %d", &esz, &d1, &d2, &d3, &k) != 5 ||
+            (esz != 4 && esz != 8) || d1 <= 0 || d2 <= 0 || d3 <= 0 || k < 0 || k >= kNumCandidates) {
+            bad = true;
+            break;
+        }
+        table[std::make_tuple(esz, (int64_t)d1, (int64_t)d2, (int64_t)d3)] = k;
+    }
+    std::fclose(f);
+    if (bad) return IPT_ERR_IO;
+    std::lock_guard<std::mutex> g(g_sel_mu);
+    for (const auto& kv : table) g_tuned[kv.first] = kv.second;
+    g_sel_cache.clear();
+    return IPT_OK;
 }
 
 int ipt_selector_active(void) { return selector_loaded() ? 1 : 0; }
@@ -286,6 +333,11 @@ ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
         if (bestk < 0 || times_us[k] < times_us[bestk]) bestk = k;
     }
     if (e != hipSuccess) set_error("ipt_autotune", e);
+    if (e == hipSuccess && st == IPT_OK && bestk >= 0) {
+        std::lock_guard<std::mutex> g(g_sel_mu);
+        g_tuned[std::make_tuple(s.esz, s.d1, s.d2, s.d3)] = bestk;
+        g_sel_cache.clear();
+    }
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
     if (stream) (void)hipStreamDestroy(stream);

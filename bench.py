@@ -35,6 +35,7 @@ import glob
 import hashlib
 import importlib
 import json
+import math
 import os
 import sqlite3
 import subprocess
@@ -81,27 +82,31 @@ MAP_LABELS = {0: ("col:c2r_pre", "MapC2RPre"), 1: ("col:c2r_post", "MapC2RPost")
               4: ("row:c2r_shuffle", "MapC2RShuffle"), 5: ("row:r2c_shuffle", "MapR2CShuffle")}
 
 
-def kernels_of(ipt, d1, d2, d3, esz):
+def kernels_of(ipt, d1, d2, d3, esz, tuned=None):
     """The kernels one transpose launches: (variant, [(label, fn(tensor), name match)])."""
     info = ipt.plan_query(esz, d1, d2, d3)
     v = info["heuristic_variant"]
+    tune = None
+    if tuned is not None:
+        v, tune = ipt.candidate(tuned["best"])
     tname = "unsigned int" if esz == 4 else "unsigned long"
     single = {ipt.VARIANT_SLICE_LDS: "k_slice_lds", ipt.VARIANT_SQUARE_SWAP: "k_square_swap",
               ipt.VARIANT_GCS: "k_gcs"}
     if v in single:
-        fn = (lambda t: ipt.transpose_variant(t, d1, d2, d3, v))
-        return v, [(ipt.VARIANT_NAMES[v], fn, f"{single[v]}<{tname}")]
-    R, C, c = info["R"], info["C"], info["c"]
+        fn = (lambda t: ipt.transpose_variant(t, d1, d2, d3, v, tune=tune))
+        return v, [(ipt.VARIANT_NAMES[v], fn, (f"{single[v]}<{tname}",))]
+    # C2R views slices as R = d2 rows x C = d1 columns, R2C as R = d1, C = d2
+    R, C = (d2, d1) if v == ipt.VARIANT_C2R else (d1, d2)
+    c = math.gcd(R, C)
     if v == ipt.VARIANT_C2R:
         maps = ([ipt.MAP_C2R_PRE] if c > 1 else []) + [ipt.MAP_C2R_SHUFFLE, ipt.MAP_C2R_POST]
     else:
         maps = [ipt.MAP_R2C_PRE, ipt.MAP_R2C_SHUFFLE] + ([ipt.MAP_R2C_POST] if c > 1 else [])
-    ix = "unsigned int" if info["index_bits"] == 32 else "unsigned long"
     out = []
     for m in maps:
         label, mname = MAP_LABELS[m]
-        out.append((label, (lambda t, m=m: ipt.run_pass(t, R, C, d3, m)),
-                    f"<{tname}, {ix}, ipt::{mname}>"))
+        out.append((label, (lambda t, m=m: ipt.run_pass(t, R, C, d3, m, tune=tune)),
+                    (f"<{tname}, ", f"ipt::{mname}>")))
     return v, out
 
 
@@ -190,7 +195,7 @@ def measure_traffic(match, shape_esz):
         if r.returncode not in (0, 139) or not dbs:
             return None, f"rocprofv3 --pmc {counter} failed (rc={r.returncode})"
         per = pmc_per_dispatch(dbs[0], counter)
-        kern = [v for k, v in per.items() if match in k]
+        kern = [v for k, v in per.items() if all(part in k for part in match)]
         # copies of both tensors appear: pick the one closest to this tensor's size
         copies = sorted((v for k, v in per.items() if "k_copy16" in k),
                         key=lambda v: abs(v * 1024 - copy_bytes))
@@ -297,6 +302,18 @@ def main():
         for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
             ipt.transpose(t, d1, d2, d3)
 
+    # Autotune the workload shapes (untimed): ipt_autotune measures every
+    # candidate and records the fastest for each exact shape, which
+    # IPT_VARIANT_AUTO then uses.
+    tuned = {}
+    for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
+        times, best = ipt.autotune(t, d1, d2, d3, reps=10)
+        tuned[f"{d1}x{d2}x{d3}x{esz}"] = {"best": best, "variant": ipt.VARIANT_NAMES[ipt.candidate(best)[0]],
+                                          "times_us": [round(float(x), 2) for x in times]}
+    # the autotune runs permuted the data: restore the iota layout
+    for t in tensors:
+        t.copy_(torch.arange(t.numel(), dtype=t.dtype, device="cuda"))
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -319,11 +336,11 @@ def main():
     # Per-kernel timing (each kernel alone, events on the launch stream).
     kernels = []
     for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
-        variant, klist = kernels_of(ipt, d1, d2, d3, esz)
+        variant, klist = kernels_of(ipt, d1, d2, d3, esz, tuned.get(f"{d1}x{d2}x{d3}x{esz}"))
         alg = 2 * t.numel() * t.element_size()
         for label, fn, match in klist:
             ms = time_calls(torch, fn, t, 20)
-            kernels.append({"kernel": label, "shape": [d1, d2, d3, esz], "match": match,
+            kernels.append({"kernel": label, "shape": [d1, d2, d3, esz], "match": list(match),
                             "variant": ipt.VARIANT_NAMES[variant], "ms": round(ms, 5),
                             "alg_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1)})
     dom = max(kernels, key=lambda k: k["ms"])
@@ -382,7 +399,9 @@ def main():
         "kernels": kernels,
         "batched": batched,
         "selector_heldout": sel_eval,
-        "dispatch": "learned selector" if ipt.selector_active() else "heuristic",
+        "dispatch": "autotuned (measured per exact shape), learned selector for others"
+                    if ipt.selector_active() else "autotuned, heuristic for others",
+        "autotune": tuned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

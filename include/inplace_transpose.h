@@ -182,6 +182,15 @@ int ipt_heuristic_candidate(int elem_bytes, int64_t d1, int64_t d2, int64_t d3);
 ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3, int reps,
                         float* times_us, int* best);
 
+/* Measured choices: every successful ipt_autotune records the fastest
+ * candidate for its exact shape; IPT_VARIANT_AUTO uses a recorded choice
+ * before the selector.  The table can be saved and loaded as text
+ * ("elem_bytes d1 d2 d3 candidate" per line; IPT_ERR_IO on a bad file, which
+ * then changes nothing). */
+ipt_status ipt_tuning_clear(void);
+ipt_status ipt_tuning_save(const char* path);
+ipt_status ipt_tuning_load(const char* path);
+
 /* Train the MLP on the GPU with the library's HIP kernels (full-batch Adam on
  * masked squared error of log2 times).  x: n x IPT_MLP_IN features; y: n x
  * IPT_NUM_CANDIDATES log2 times; mask: n x IPT_NUM_CANDIDATES, 1 where y is

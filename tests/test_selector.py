@@ -178,3 +178,44 @@ def test_learned_dispatch_is_bit_exact(ipt, oracle):
         print("[selector] candidates used", sorted(used))
     finally:
         ipt.selector_load(None)
+
+
+def test_tuning_table_roundtrip(ipt, tmp_path):
+    path = tmp_path / "table.txt"
+    path.write_text("# ipt tuning table: elem_bytes d1 d2 d3 candidate\n4 777 333 2 5\n8 64 64 1 1\n")
+    ipt.tuning_clear()
+    try:
+        ipt.tuning_load(str(path))
+        assert ipt.select_candidate(4, 777, 333, 2) == 5
+        assert ipt.select_candidate(8, 64, 64, 1) == 1
+        out = tmp_path / "saved.txt"
+        ipt.tuning_save(str(out))
+        assert "4 777 333 2 5" in out.read_text()
+        bad = tmp_path / "bad.txt"
+        bad.write_text("4 10 10 1 99\n")
+        with pytest.raises(ipt.TransposeError):
+            ipt.tuning_load(str(bad))
+        with pytest.raises(ipt.TransposeError):
+            ipt.tuning_load(str(tmp_path / "missing.txt"))
+        assert ipt.select_candidate(4, 777, 333, 2) == 5  # unchanged by the failed loads
+    finally:
+        ipt.tuning_clear()
+
+
+@pytest.mark.gpu
+def test_autotune_records_measured_choice(ipt, oracle):
+    import torch
+    ipt.tuning_clear()
+    try:
+        d1, d2, d3, esz = 384, 320, 6, 4
+        a = oracle.random_bits(d1 * d2 * d3, esz, 1200)
+        t = torch.from_numpy(a.view(np.int32)).cuda()
+        times, best = ipt.autotune(t, d1, d2, d3, reps=3)
+        assert times[best] > 0 and all(x < 0 or x >= times[best] for x in times)
+        assert ipt.select_candidate(esz, d1, d2, d3) == best
+        t = torch.from_numpy(a.view(np.int32)).cuda()
+        ipt.transpose(t, d1, d2, d3)
+        torch.cuda.synchronize()
+        assert np.array_equal(t.cpu().numpy().view(a.dtype), oracle.seq_tt_213(a, d1, d2, d3))
+    finally:
+        ipt.tuning_clear()
