@@ -257,8 +257,9 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
                 for (uint32_t m = 0; m < gg.g; m++) gcs_items.push_back({idx, m, sl});
             b->info.n_gcs_items += (uint64_t)gg.g * s.d3;
         } else {
-            I.kind = BK_SWAP;
             const uint64_t TS = s.esz == 4 ? 64 : 32;
+            I.kind = ((uint64_t)s.d1 * s.esz) % 16 == 0 && (uintptr_t)d.data % 16 == 0 ? BK_SWAP_VEC
+                                                                                        : BK_SWAP;
             const uint64_t nt = ((uint64_t)s.d1 + TS - 1) / TS;
             const uint64_t pairs = nt * (nt + 1) / 2;
             if (pairs >= ((uint64_t)1 << 32)) {

@@ -92,7 +92,8 @@ GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spr
     const int64_t nti = (R + gg.ts - 1) / gg.ts, ntj = (C + gg.ts - 1) / gg.ts;
     const int64_t ntiles = nti * ntj;
     const int64_t klmax = cap / tile_bytes;
-    const int64_t krmax = std::min<int64_t>(klmax, (int64_t)kGcsRV * kGcsBlock / vpt);
+    const int64_t rv = gg.vec ? kGcsRV : kGcsRVScalar;
+    const int64_t krmax = std::min<int64_t>(klmax, rv * kGcsBlock / vpt);
     gg.ok = klmax >= 1 && ntiles < ((int64_t)1 << 31) && R * C < ((int64_t)1 << 40);
     if (!gg.ok) return gg;
     const int64_t ncu = ncu_of(ctx);
@@ -200,10 +201,11 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
     const uint32_t N = (uint32_t)s.d1;
     const uint32_t nt = (N + TS - 1) / TS;
     const uint64_t pairs = (uint64_t)nt * (nt + 1) / 2;
-    auto k = k_square_swap<T, TS>;
+    const bool vec = ((uint64_t)N * sizeof(T)) % 16 == 0 && (uintptr_t)data % 16 == 0;
+    auto k = vec ? k_square_swap<T, TS, true> : k_square_swap<T, TS, false>;
     int grid;
-    grid_for(ctx, k, 256, 0, pairs * (uint64_t)s.d3, t.grid_mult, &grid);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, data, N, pairs, (uint64_t)s.d3);
+    grid_for(ctx, k, kSwapBlock, 0, pairs * (uint64_t)s.d3, t.grid_mult, &grid);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kSwapBlock), 0, st, data, N, pairs, (uint64_t)s.d3);
     IPT_LAUNCH_CHECK("k_square_swap");
     return IPT_OK;
 }
@@ -213,7 +215,7 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
 template <typename T, int TS, bool VEC>
 static ipt_status launch_gcs(DeviceCtx* ctx, T* data, const Shape& s, const GcsGeom& gg,
                              const Tune& t, hipStream_t st) {
-    auto k = k_gcs<T, TS, VEC, kGcsBlock, kGcsRV>;
+    auto k = k_gcs<T, TS, VEC, kGcsBlock, VEC ? kGcsRV : kGcsRVScalar>;
     ipt_status stt = allow_lds(k, gg.lds);
     if (stt != IPT_OK) return stt;
     int per_cu = 0;

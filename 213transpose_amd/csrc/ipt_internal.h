@@ -80,7 +80,10 @@ ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, i
                     int map_id, const Tune& t, hipStream_t stream);
 
 constexpr int kGcsBlock = 512;  // 8 waves per workgroup
-constexpr int kGcsRV = 8;       // 16-byte vectors per thread held in registers across the barrier
+// Elements each thread holds in registers across the barrier: 8 16-byte
+// vectors on the vector path, 64 elements on the scalar path.
+constexpr int kGcsRV = 8;
+constexpr int kGcsRVScalar = 64;
 
 struct GcsGeom {
     int ts;           // tile side

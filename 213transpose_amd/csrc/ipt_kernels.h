@@ -95,43 +95,82 @@ k_slice_lds(T* __restrict__ data, uint32_t R, uint32_t C, uint32_t P, uint64_t n
 // ---------------------------------------------------------------------------
 // Square slices N x N: swap-transpose tile pairs (I,J) / (J,I), I <= J.
 // One sweep; both tiles are staged in LDS (2 * TS * (TS+1) elements at lds)
-// before either is written.  Pair t = J(J+1)/2 + I.
-template <typename T, int TS>
+// before either is written.  Pair t = J(J+1)/2 + I.  VEC: 16-byte global
+// loads / stores (N*esz a multiple of 16 and a 16-byte aligned base); both
+// tiles' loads are in flight before the first LDS store.
+template <typename T, int TS, int BLOCK, bool VEC>
 __device__ __forceinline__ void swap_body(T* __restrict__ base, uint32_t N, uint64_t t, T* lds) {
-    T* A = lds;
-    T* B = lds + TS * (TS + 1);
     constexpr uint32_t TP = TS + 1;
-    const uint32_t tx = threadIdx.x % TS, ty = threadIdx.x / TS;
-    const uint32_t rows = blockDim.x / TS;
+    constexpr uint32_t PER = VEC ? 16 / sizeof(T) : 1;
+    constexpr uint32_t VPR = TS / PER;           // vectors per tile row
+    constexpr uint32_t VPT = TS * VPR;           // vectors per tile
+    constexpr uint32_t UPT = (VPT + BLOCK - 1) / BLOCK;
+    using V = typename std::conditional<VEC, uint4, T>::type;
+    T* A = lds;
+    T* B = lds + TS * TP;
     uint64_t J = (uint64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
     while (J * (J + 1) / 2 > t) J--;
     while ((J + 1) * (J + 2) / 2 <= t) J++;
     const uint64_t I = t - J * (J + 1) / 2;
     const uint32_t r0 = (uint32_t)I * TS, c0 = (uint32_t)J * TS;
-    for (uint32_t y = ty; y < TS; y += rows) {
-        uint32_t r = r0 + y, c = c0 + tx;
-        if (r < N && c < N) A[y * TP + tx] = base[(uint64_t)r * N + c];
-        uint32_t r2 = c0 + y, c2 = r0 + tx;
-        if (I != J && r2 < N && c2 < N) B[y * TP + tx] = base[(uint64_t)r2 * N + c2];
+    const bool diag = I == J;
+    V ra[UPT], rb[UPT];
+#pragma unroll
+    for (uint32_t u = 0; u < UPT; u++) {
+        const uint32_t v = u * BLOCK + threadIdx.x;
+        const uint32_t y = v / VPR, x = (v % VPR) * PER;
+        if (v < VPT) {
+            if (r0 + y < N && c0 + x < N) ra[u] = *reinterpret_cast<const V*>(base + (uint64_t)(r0 + y) * N + c0 + x);
+            if (!diag && c0 + y < N && r0 + x < N)
+                rb[u] = *reinterpret_cast<const V*>(base + (uint64_t)(c0 + y) * N + r0 + x);
+        }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < UPT; u++) {
+        const uint32_t v = u * BLOCK + threadIdx.x;
+        const uint32_t y = v / VPR, x = (v % VPR) * PER;
+        if (v < VPT) {
+            const T* ea = reinterpret_cast<const T*>(&ra[u]);
+            const T* eb = reinterpret_cast<const T*>(&rb[u]);
+#pragma unroll
+            for (uint32_t q = 0; q < PER; q++) {
+                A[y * TP + x + q] = ea[q];
+                if (!diag) B[y * TP + x + q] = eb[q];
+            }
+        }
     }
     __syncthreads();
-    for (uint32_t y = ty; y < TS; y += rows) {
-        // tile (I,J) <- transpose of tile (J,I); diagonal: of itself.
-        uint32_t r = r0 + y, c = c0 + tx;
-        if (r < N && c < N) base[(uint64_t)r * N + c] = (I == J) ? A[tx * TP + y] : B[tx * TP + y];
-        uint32_t r2 = c0 + y, c2 = r0 + tx;
-        if (I != J && r2 < N && c2 < N) base[(uint64_t)r2 * N + c2] = A[tx * TP + y];
+#pragma unroll
+    for (uint32_t u = 0; u < UPT; u++) {
+        const uint32_t v = u * BLOCK + threadIdx.x;
+        const uint32_t y = v / VPR, x = (v % VPR) * PER;  // output row y, cols x..x+PER
+        if (v < VPT) {
+            // tile (I,J) <- transpose of tile (J,I) (of itself on the diagonal)
+            union { V v; T e[PER]; } o;
+            const T* src = diag ? A : B;
+#pragma unroll
+            for (uint32_t q = 0; q < PER; q++) o.e[q] = src[(x + q) * TP + y];
+            if (r0 + y < N && c0 + x < N) *reinterpret_cast<V*>(base + (uint64_t)(r0 + y) * N + c0 + x) = o.v;
+            if (!diag) {
+#pragma unroll
+                for (uint32_t q = 0; q < PER; q++) o.e[q] = A[(x + q) * TP + y];
+                if (c0 + y < N && r0 + x < N)
+                    *reinterpret_cast<V*>(base + (uint64_t)(c0 + y) * N + r0 + x) = o.v;
+            }
+        }
     }
 }
 
-template <typename T, int TS>
-__global__ void __launch_bounds__(256)
+constexpr int kSwapBlock = 256;
+
+template <typename T, int TS, bool VEC>
+__global__ void __launch_bounds__(kSwapBlock)
 k_square_swap(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3) {
     __shared__ T tiles[2 * TS * (TS + 1)];
     const uint64_t total = pairs * d3;
     for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const uint64_t sl = w / pairs;
-        swap_body<T, TS>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles);
+        swap_body<T, TS, kSwapBlock, VEC>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles);
         __syncthreads();
     }
 }
@@ -496,7 +535,8 @@ namespace ipt {
 // Workgroup b takes items b, b + grid, ... (static round-robin), so the
 // deadlock argument of k_gcs holds unchanged (every g <= grid, other items
 // never wait).
-enum BatchKind : uint32_t { BK_SLICE = 0, BK_GCS_VEC = 1, BK_GCS_SCALAR = 2, BK_SWAP = 3 };
+enum BatchKind : uint32_t { BK_SLICE = 0, BK_GCS_VEC = 1, BK_GCS_SCALAR = 2, BK_SWAP = 3,
+                            BK_SWAP_VEC = 4 };
 
 struct BatchInst {
     uint64_t data;        // device address of slice 0
@@ -532,12 +572,15 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
                                                   ctl, lds);
         break;
     case BK_GCS_SCALAR:
-        gcs_body<T, TS_GCS, false, kBatchBlock, 8>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+        gcs_body<T, TS_GCS, false, kBatchBlock, 64>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
                                                    I.ntiles, it.sub,
                                                    bars + I.bar + bar_words(I.g) * it.slice, ctl, lds);
         break;
+    case BK_SWAP_VEC:
+        swap_body<T, TS_SWAP, kBatchBlock, true>(base, I.R, it.sub, lds);
+        break;
     default:
-        swap_body<T, TS_SWAP>(base, I.R, it.sub, lds);
+        swap_body<T, TS_SWAP, kBatchBlock, false>(base, I.R, it.sub, lds);
         break;
     }
 }
