@@ -102,6 +102,25 @@ ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr) {
     return IPT_OK;
 }
 
+// Event to attach to the next launch that uses the arena on stream s (the
+// launch completes it, as arena_release's record would); null while s is
+// being captured, where the caller records with arena_release instead.
+ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev) {
+    *ev = nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(s, &cs);
+    if (e != hipSuccess) { set_error("hipStreamIsCapturing", e); return IPT_ERR_HIP; }
+    if (cs != hipStreamCaptureStatusNone) return IPT_OK;
+    if (!a.ev) {
+        e = hipEventCreateWithFlags(&a.ev, hipEventDisableTiming);
+        if (e != hipSuccess) { set_error("hipEventCreate", e); return IPT_ERR_HIP; }
+    }
+    a.ev_valid = true;
+    a.last = s;
+    *ev = a.ev;
+    return IPT_OK;
+}
+
 ipt_status arena_release(Arena& a, hipStream_t s) {
     if (!a.ev) {
         hipError_t e = hipEventCreateWithFlags(&a.ev, hipEventDisableTiming);

@@ -242,9 +242,8 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             }
             b->info.n_slice_items += ((uint64_t)s.d3 + per - 1) / per;
         } else if (p == Route::Gcs) {
-            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false);
-            const bool vec = gg.vec && ((uintptr_t)d.data % 16) == 0;
-            I.kind = vec ? BK_GCS_VEC : BK_GCS_SCALAR;
+            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, ((uintptr_t)d.data % 16) == 0);
+            I.kind = gg.vec ? BK_GCS_VEC : BK_GCS_SCALAR;
             I.g = gg.g;
             I.K = gg.K;
             I.KL = gg.KL;

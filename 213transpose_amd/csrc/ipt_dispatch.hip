@@ -12,7 +12,13 @@
 #include "ipt_internal.h"
 #include "ipt_kernels.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <set>
+#include <tuple>
 
 namespace ipt {
 
@@ -77,23 +83,29 @@ static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
 // ---------------------------------------------------------------- GCS geometry
 
 // Geometry of the cooperative kernel.  Each member holds at most
-// KLmax tiles in LDS plus KRmax in registers.  When every slice fits in one
-// round of one workgroup per CU, slices are spread over all CUs (larger g,
-// less data per member); otherwise members are filled to capacity.
-GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread) {
+// KLmax tiles in LDS plus KRmax in registers: on the vector path unpadded
+// wave tiles, JL per wave in LDS and JR per wave in registers; on the scalar
+// path padded tiles, the register-held ones staged through LDS slots after
+// the first round of stores (so KRmax <= KLmax).  When every slice fits in
+// one round of one workgroup per CU, slices are spread over all CUs (larger
+// g, less data per member); otherwise members are filled to capacity.
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread,
+                 bool aligned) {
     GcsGeom gg{};
     const int64_t R = s.d2, C = s.d1, esz = s.esz;
     gg.ts = esz == 4 ? 32 : 16;
-    gg.vec = (R * esz) % 16 == 0 && (C * esz) % 16 == 0;
-    const int64_t per = gg.vec ? 16 / esz : 1;
-    const int64_t vpt = (int64_t)gg.ts * gg.ts / per;
-    const int64_t tile_bytes = (int64_t)gg.ts * (gg.ts + 1) * esz;
+    gg.vec = aligned && (R * esz) % 16 == 0 && (C * esz) % 16 == 0 &&
+             R * C * esz < ((int64_t)1 << 31);  // 32-bit buffer offsets
+    const int64_t elems = (int64_t)gg.ts * gg.ts;
+    const int64_t tile_bytes = (gg.vec ? elems : (int64_t)gg.ts * (gg.ts + 1)) * esz;
     const int64_t cap = std::min<int64_t>(lds_cap, lds_max_of(ctx)) - 64;  // static LDS headroom
     const int64_t nti = (R + gg.ts - 1) / gg.ts, ntj = (C + gg.ts - 1) / gg.ts;
     const int64_t ntiles = nti * ntj;
-    const int64_t klmax = cap / tile_bytes;
-    const int64_t rv = gg.vec ? kGcsRV : kGcsRVScalar;
-    const int64_t krmax = std::min<int64_t>(klmax, rv * kGcsBlock / vpt);
+    constexpr int64_t nw = kGcsBlock / 64;
+    const int64_t klmax = gg.vec ? std::min<int64_t>(cap / tile_bytes, gcs_wave_lds_tiles(esz) * nw)
+                                 : cap / tile_bytes;
+    const int64_t krmax = gg.vec ? gcs_wave_reg_tiles(esz) * nw
+                                 : std::min<int64_t>(klmax, gcs_scalar_rv(esz) * kGcsBlock / elems);
     gg.ok = klmax >= 1 && ntiles < ((int64_t)1 << 31) && R * C < ((int64_t)1 << 40);
     if (!gg.ok) return gg;
     const int64_t ncu = ncu_of(ctx);
@@ -138,26 +150,54 @@ int choose_variant(const DeviceCtx* ctx, const Shape& s) {
     return s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
 }
 
+// Occupancy and the dynamic-LDS attribute belong to a kernel on a device and
+// do not change; both are cached, since querying or setting them costs host
+// time on every call.
+static std::mutex g_kern_mu;
+static std::map<std::tuple<int, const void*, int, size_t>, int> g_occupancy;
+static std::set<std::pair<int, const void*>> g_lds_raised;
+
+static int occupancy(const DeviceCtx* ctx, const void* fn, int block, size_t lds) {
+    const auto key = std::make_tuple(ctx->dev, fn, block, lds);
+    {
+        std::lock_guard<std::mutex> g(g_kern_mu);
+        auto it = g_occupancy.find(key);
+        if (it != g_occupancy.end()) return it->second;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) != hipSuccess ||
+        per_cu <= 0) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    std::lock_guard<std::mutex> g(g_kern_mu);
+    g_occupancy[key] = per_cu;
+    return per_cu;
+}
+
 // Workgroups for a grid-stride kernel: occupancy x CUs x grid_mult, capped by
 // the number of work items.
 template <typename K>
 static ipt_status grid_for(const DeviceCtx* ctx, K kernel, int block, size_t lds, uint64_t items,
                            int grid_mult, int* grid) {
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds);
-    if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+    const int per_cu = occupancy(ctx, reinterpret_cast<const void*>(kernel), block, lds);
     uint64_t g = (uint64_t)per_cu * (uint64_t)ncu_of(ctx) * (uint64_t)std::max(grid_mult, 1);
     g = std::min<uint64_t>(g, items);
     *grid = (int)std::max<uint64_t>(g, 1);
     return IPT_OK;
 }
 
+// Dynamic LDS above 64 KB must be allowed per kernel; the first use on a
+// device raises the limit to the device's maximum once.
 template <typename K>
-static ipt_status allow_lds(K kernel, size_t lds) {
-    if (lds > 64 * 1024) {
-        IPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    }
+static ipt_status allow_lds(const DeviceCtx* ctx, K kernel, size_t lds) {
+    if (lds <= 64 * 1024) return IPT_OK;
+    const void* fn = reinterpret_cast<const void*>(kernel);
+    std::lock_guard<std::mutex> g(g_kern_mu);
+    if (g_lds_raised.count({ctx->dev, fn})) return IPT_OK;
+    IPT_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)std::max<int64_t>((int64_t)lds, lds_max_of(ctx))));
+    g_lds_raised.insert({ctx->dev, fn});
     return IPT_OK;
 }
 
@@ -181,7 +221,7 @@ static ipt_status run_slice_lds(DeviceCtx* ctx, T* data, const Shape& s, const T
     per = (uint32_t)std::min<uint64_t>(per, (uint64_t)s.d3);
     const size_t lds = (size_t)per * slice_bytes;
     auto k = k_slice_lds<T>;
-    ipt_status stt = allow_lds(k, lds);
+    ipt_status stt = allow_lds(ctx, k, lds);
     if (stt != IPT_OK) return stt;
     const uint64_t items = ((uint64_t)s.d3 + per - 1) / per;
     int grid;
@@ -212,27 +252,25 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
 
 // ---------------------------------------------------------------- GCS
 
-template <typename T, int TS, bool VEC>
-static ipt_status launch_gcs(DeviceCtx* ctx, T* data, const Shape& s, const GcsGeom& gg,
-                             const Tune& t, hipStream_t st) {
-    auto k = k_gcs<T, TS, VEC, kGcsBlock, VEC ? kGcsRV : kGcsRVScalar>;
-    ipt_status stt = allow_lds(k, gg.lds);
+// Signature shared by both GCS kernels (k_gcs2: vector path, k_gcs: scalar).
+template <typename T>
+using GcsKernel = void (*)(T*, uint32_t, uint32_t, uint64_t, uint32_t, uint32_t, uint32_t, Div32,
+                           uint32_t, uint32_t*, uint32_t*);
+
+template <typename T>
+static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& s,
+                             const GcsGeom& gg, hipStream_t st) {
+    ipt_status stt = allow_lds(ctx, k, gg.lds);
     if (stt != IPT_OK) return stt;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kGcsBlock, gg.lds) != hipSuccess ||
-        per_cu < 1) {
-        (void)hipGetLastError();
-        per_cu = 1;
-    }
+    const int per_cu = occupancy(ctx, reinterpret_cast<const void*>(k), kGcsBlock, gg.lds);
     const uint64_t slots = (uint64_t)s.d3 * gg.g;
     const uint64_t resident = (uint64_t)per_cu * (uint64_t)ncu_of(ctx);
     // Never more workgroups than can be resident at once: every member of a
     // group must be running while the others wait for it.
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(slots, resident));
-    (void)t;
     if ((uint64_t)grid < gg.g) return IPT_ERR_UNSUPPORTED;  // a group must be co-resident
     if (gg.g == 1) {  // whole slices per workgroup: no barrier state needed
-        hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, data, (uint32_t)s.d2,
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, d, (uint32_t)s.d2,
                            (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL, Div32::make(gg.ntj),
                            gg.ntiles, (uint32_t*)nullptr, (uint32_t*)nullptr);
         IPT_LAUNCH_CHECK("k_gcs");
@@ -244,7 +282,20 @@ static ipt_status launch_gcs(DeviceCtx* ctx, T* data, const Shape& s, const GcsG
                         st, &sync);
     if (stt != IPT_OK) return stt;
     uint32_t* ctl = (uint32_t*)sync;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, data, (uint32_t)s.d2,
+    // The arena's ordering event rides on the launch itself (a separate
+    // event record between kernels costs ~3 us of queue time per call);
+    // during stream capture it is recorded as its own node.
+    hipEvent_t ev = nullptr;
+    stt = arena_event(ctx->sync, st, &ev);
+    if (stt != IPT_OK) return stt;
+    if (ev) {
+        hipExtLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, nullptr, ev, 0, d,
+                              (uint32_t)s.d2, (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL,
+                              Div32::make(gg.ntj), gg.ntiles, ctl, ctl + 32);
+        IPT_LAUNCH_CHECK("k_gcs");
+        return IPT_OK;
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, d, (uint32_t)s.d2,
                        (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL, Div32::make(gg.ntj),
                        gg.ntiles, ctl, ctl + 32);
     IPT_LAUNCH_CHECK("k_gcs");
@@ -253,12 +304,11 @@ static ipt_status launch_gcs(DeviceCtx* ctx, T* data, const Shape& s, const GcsG
 
 template <typename T>
 static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t, hipStream_t st) {
-    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap);
+    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, ((uintptr_t)data % 16) == 0);
     if (!gg.ok) return IPT_ERR_UNSUPPORTED;
-    const bool vec = gg.vec && ((uintptr_t)data % 16) == 0;
     constexpr int TS = sizeof(T) == 4 ? 32 : 16;
-    return vec ? launch_gcs<T, TS, true>(ctx, data, s, gg, t, st)
-               : launch_gcs<T, TS, false>(ctx, data, s, gg, t, st);
+    if (gg.vec) return launch_gcs<T>(ctx, k_gcs2<T, kGcsBlock>, data, s, gg, st);
+    return launch_gcs<T>(ctx, k_gcs<T, TS, kGcsBlock, gcs_scalar_rv(sizeof(T))>, data, s, gg, st);
 }
 
 // ---------------------------------------------------------------- passes
@@ -282,7 +332,7 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
         const uint32_t nstrips = (uint32_t)((C + W - 1) / W);
         const size_t lds = (size_t)(R * W * esz);
         auto k = k_col_gather_lds<T, Ix, Map>;
-        ipt_status stt = allow_lds(k, lds);
+        ipt_status stt = allow_lds(ctx, k, lds);
         if (stt != IPT_OK) return stt;
         int grid;
         grid_for(ctx, k, t.block, lds, d3 * nstrips, t.grid_mult, &grid);
@@ -326,7 +376,7 @@ static ipt_status row_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
         per = (uint32_t)std::min<uint64_t>(per, nrows);
         const size_t lds = (size_t)(per * row_bytes + 16);
         auto k = k_row_gather_lds<T, Ix, Map>;
-        ipt_status stt = allow_lds(k, lds);
+        ipt_status stt = allow_lds(ctx, k, lds);
         if (stt != IPT_OK) return stt;
         int grid;
         grid_for(ctx, k, t.block, lds, (nrows + per - 1) / per, t.grid_mult, &grid);

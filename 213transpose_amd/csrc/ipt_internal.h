@@ -58,6 +58,7 @@ constexpr size_t kScratchCap = (size_t)256 << 20;  // fallback-tier scratch boun
 void set_error(const char* what, hipError_t e);
 ipt_status get_ctx(DeviceCtx** out);
 ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr);
+ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev);
 ipt_status arena_release(Arena& a, hipStream_t s);
 ipt_status validate(const void* data, int esz, int64_t d1, int64_t d2, int64_t d3);
 Shape make_shape(int esz, int64_t d1, int64_t d2, int64_t d3);
@@ -80,10 +81,6 @@ ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, i
                     int map_id, const Tune& t, hipStream_t stream);
 
 constexpr int kGcsBlock = 512;  // 8 waves per workgroup
-// Elements each thread holds in registers across the barrier: 8 16-byte
-// vectors on the vector path, 64 elements on the scalar path.
-constexpr int kGcsRV = 8;
-constexpr int kGcsRVScalar = 64;
 
 struct GcsGeom {
     int ts;           // tile side
@@ -97,7 +94,12 @@ struct GcsGeom {
 };
 
 // spread: when every slice fits one round, use more members per slice.
-GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true);
+// aligned: the data pointer is 16-byte aligned (the vector path needs it as
+// well as both dimensions being multiples of 16 bytes).  The scalar path
+// holds at least as many tiles per member as the vector path, so a shape
+// feasible with aligned data is feasible without.
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true,
+                 bool aligned = true);
 int64_t slice_pitch(int64_t C);
 int64_t lds_max_of(const DeviceCtx* ctx);
 int ncu_of(const DeviceCtx* ctx);

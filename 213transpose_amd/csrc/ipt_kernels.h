@@ -299,9 +299,9 @@ namespace ipt {
 // one workgroup's LDS.  Slot s is member s % g of the group that owns slice
 // s / g; workgroup b takes slots b, b + grid, b + 2*grid, ... (static
 // round-robin, no global counter).  Each member stages its share of the
-// slice's TS x TS tiles in LDS, the group meets at a per-slice arrival
-// counter, then every member writes its tiles transposed.  All reads of a
-// slice complete before any write to it and slices never overlap, so no
+// slice's tiles (LDS plus registers), the group meets at a per-slice
+// arrival counter, then every member writes its tiles transposed.  All reads
+// of a slice complete before any write to it and slices never overlap, so no
 // workgroup ever reads data another one wrote: the counters carry no data and
 // relaxed agent-scope atomics suffice.
 //
@@ -340,20 +340,28 @@ __device__ __forceinline__ bool spin_until(uint32_t* ctr, uint32_t target) {
     return true;
 }
 
-// Group barrier of g members over kBarLines 128-byte lines (one slice):
-//   line 0: word 0 top counter (sub-counters completed), word 16 departures
-//   line 1 + q: word 0 arrivals of sub-group q (members m with m % nsub == q)
-// With nsub = min(g, kBarLines - 1) sub-counters, at most ceil(g / nsub)
-// members contend on a line.  The last arriver of a sub-group bumps the top
-// counter; members wait for the top counter to reach nsub.  The last member
-// to depart zeroes every word it used (zero at rest).
+// Wait until every vector-memory load this wave issued has returned its data.
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Group barrier of g members over kBarLines 128-byte lines (one slice).
+// Up to kBarLines - 1 members: one arrival counter (line 0, word 0), each
+// member adds 1 and waits for g.  More members: line 1 + q, word 0 counts
+// the arrivals of sub-group q (members m with m % nsub == q, nsub =
+// kBarLines - 1); the last arriver of a sub-group bumps line 0 word 0 and
+// members wait for it to reach nsub.  Either way line 0 word 16 counts
+// departures and the last member to depart zeroes every word used (zero at
+// rest).
 constexpr uint32_t kBarLines = 17;
 
 // Words of barrier state per slice for a group of g.
 IPT_HD uint64_t bar_words(uint32_t g) { return 32ull * (1 + (g < kBarLines - 1 ? g : kBarLines - 1)); }
 
 __device__ __forceinline__ bool group_arrive_wait(uint32_t* bar, uint32_t m, uint32_t g) {
-    const uint32_t nsub = min(g, kBarLines - 1);
+    if (g <= kBarLines - 1) {
+        atomic_add_agent(&bar[0], 1u);
+        return spin_until(&bar[0], g);
+    }
+    const uint32_t nsub = kBarLines - 1;
     const uint32_t q = m % nsub;
     const uint32_t members = g / nsub + (q < g % nsub ? 1u : 0u);
     if (atomic_add_agent(&bar[32 * (1 + q)], 1u) == members - 1) atomic_add_agent(&bar[0], 1u);
@@ -362,56 +370,246 @@ __device__ __forceinline__ bool group_arrive_wait(uint32_t* bar, uint32_t m, uin
 
 __device__ __forceinline__ void group_depart(uint32_t* bar, uint32_t g) {
     if (atomic_add_agent(&bar[16], 1u) == g - 1) {
-        const uint32_t nsub = min(g, kBarLines - 1);
-        for (uint32_t q = 0; q < nsub; q++) atomic_store_agent(&bar[32 * (1 + q)], 0u);
+        if (g > kBarLines - 1)
+            for (uint32_t q = 0; q < kBarLines - 1; q++) atomic_store_agent(&bar[32 * (1 + q)], 0u);
         atomic_store_agent(&bar[0], 0u);
         atomic_store_agent(&bar[16], 0u);
     }
 }
 
-template <typename T, int TS, bool VEC>
-struct GcsTile {
-    static constexpr uint32_t TP = TS + 1;     // padded LDS pitch: transposed reads hit distinct banks
-    static constexpr uint32_t TILE = TS * TP;  // elements per staged tile
-    static constexpr uint32_t PER = VEC ? 16 / sizeof(T) : 1;
-    static constexpr uint32_t VPR = TS / PER;  // vectors per tile row
-    static constexpr uint32_t VPT = TS * VPR;  // vectors per tile
-    using V = typename std::conditional<VEC, uint4, T>::type;
+// Staging shared by the members of one slice: after every wave has its
+// loads back, thread 0 arrives and waits for the group; `bar` is null when
+// the slice has one member.
+__device__ __forceinline__ void group_sync(uint32_t* bar, uint32_t* ctl, uint32_t m, uint32_t g) {
+    wait_vmem();
+    __syncthreads();
+    if (g > 1) {
+        if (threadIdx.x == 0 && !group_arrive_wait(bar, m, g)) atomic_store_agent(&ctl[2], 1u);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vector path (both dimensions multiples of 16 bytes, 16-byte aligned data,
+// slice below 2^31 bytes).
+//
+// Global accesses go through a raw buffer resource over the slice: a load at
+// an offset outside it returns zero without touching memory and a store
+// outside it is dropped.  Lanes with nothing to move use kOutside, so the
+// code has no branches around its memory operations and every load of a
+// wave is in flight at once.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+constexpr uint32_t kOutside = 0x80000000u;  // past every resource built here (< 2^31 bytes)
+
+// Cache policy of the transposing stores: sc1 (device scope) writes through
+// the XCD's L2, so a launch ends with no dirty lines for the end-of-kernel
+// write-back and the next launch starts about 4 us sooner (measured on the
+// [512][384][64] f32 permute: 24.4 -> 23.0 us per back-to-back launch).
+constexpr int kStoreDevice = 0x10;
+
+// Raw buffer resource over [base, base + bytes); word 3 = 0x00020000 is the
+// gfx9 raw-buffer format (32-bit data, no swizzle, range-checked).
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+// Wave tile: one wavefront moves a TS x TS tile, TS = 8 * PER with PER =
+// 16 / sizeof(T) (32 x 32 for 4-byte, 16 x 16 for 8-byte elements), as 64
+// lane micro-tiles of PER x PER.  Lane (a, b) = (lane / 8, lane % 8) loads
+// rows PER*a + k (k < PER) at columns PER*b .. PER*b + PER - 1, so eight
+// lanes cover one 128-byte row segment; it transposes its micro-tile in
+// registers and stores output rows PER*b + q at columns PER*a .., eight lanes
+// again covering one 128-byte segment.  Since both dimensions are multiples
+// of PER, a micro-tile lies wholly inside or wholly outside the matrix.
+template <typename T>
+struct WaveTile {
+    static constexpr uint32_t PER = 16 / sizeof(T);
+    static constexpr uint32_t TS = 8 * PER;
+    u32x4 v[PER];
+
+    // Tile with top-left (r0, c0) of a rows x cols row-major slice; `live`
+    // false loads nothing (zeros).
+    __device__ __forceinline__ void load(Rsrc rs, uint32_t r0, uint32_t c0, uint32_t rows,
+                                         uint32_t cols, uint32_t lane, bool live) {
+        const uint32_t r = r0 + PER * (lane >> 3), c = c0 + PER * (lane & 7);
+        const bool ok = live && r < rows && c < cols;
+        const uint32_t o = (r * cols + c) * (uint32_t)sizeof(T);
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++)
+            v[k] = __builtin_amdgcn_raw_buffer_load_b128(
+                rs, ok ? o + k * cols * (uint32_t)sizeof(T) : kOutside, 0, 0);
+    }
+    __device__ __forceinline__ void transpose() {
+        union U { u32x4 v; T e[PER]; };
+        U in[PER], out[PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) in[k].v = v[k];
+#pragma unroll
+        for (uint32_t q = 0; q < PER; q++)
+#pragma unroll
+            for (uint32_t k = 0; k < PER; k++) out[q].e[k] = in[k].e[q];
+#pragma unroll
+        for (uint32_t q = 0; q < PER; q++) v[q] = out[q].v;
+    }
+    // After transpose(): the tile loaded at (r0, c0) of a rows x cols slice
+    // goes to output rows c0 + .., columns r0 + .. (the output has `rows`
+    // columns); `live` false stores nothing.
+    __device__ __forceinline__ void store_t(Rsrc rs, uint32_t r0, uint32_t c0, uint32_t rows,
+                                            uint32_t cols, uint32_t lane, bool live) const {
+        const uint32_t orow = c0 + PER * (lane & 7), ocol = r0 + PER * (lane >> 3);
+        const bool ok = live && orow < cols && ocol < rows;
+        const uint32_t o = (orow * rows + ocol) * (uint32_t)sizeof(T);
+#pragma unroll
+        for (uint32_t q = 0; q < PER; q++)
+            __builtin_amdgcn_raw_buffer_store_b128(
+                v[q], rs, ok ? o + q * rows * (uint32_t)sizeof(T) : kOutside, 0, kStoreDevice);
+    }
 };
 
-// Global address of vector v of tile t (input layout); false when outside.
-template <typename T, int TS, bool VEC>
+// Wave tiles a wave stages in LDS (JL) and holds in registers (JR) on the
+// vector path, per element size: 40 / 80 LDS tiles fill 160 KB, and the
+// register tiles hold 64 KB per 512-thread workgroup.
+constexpr int gcs_wave_lds_tiles(int esz) { return esz == 4 ? 5 : 10; }
+constexpr int gcs_wave_reg_tiles(int esz) { return esz == 4 ? 2 : 4; }
+
+// One vector-path GCS slot: member m of the group of g that transposes the
+// slice at `base`.  Member m owns tiles t = m + k*g, k < nk; wave w of the
+// workgroup takes k = w, w + NW, ...  Tiles k < kl are staged in LDS
+// (already transposed, lane-linear: conflict-free 16-byte accesses, and
+// each wave reads back only what it wrote); tiles kl <= k < nk stay in
+// registers across the group barrier.
+template <typename T, int BLOCK>
+__device__ __forceinline__ void gcs2_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
+                                          uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
+                                          uint32_t m, uint32_t* bar, uint32_t* ctl, u32x4* lds) {
+    using W = WaveTile<T>;
+    constexpr uint32_t NW = BLOCK / 64;
+    constexpr int JL = gcs_wave_lds_tiles(sizeof(T));
+    constexpr int JR = gcs_wave_reg_tiles(sizeof(T));
+    constexpr uint32_t TV = 64 * W::PER;  // 16-byte vectors per staged tile
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t nk = m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0;
+    const uint32_t kl = min(nk, KL);
+    const Rsrc rs = make_rsrc(base, R * C * (uint32_t)sizeof(T));
+    auto origin = [&](uint32_t k, uint32_t* r0, uint32_t* c0) {
+        const uint32_t t = m + k * g;
+        const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
+        *r0 = ti * W::TS;
+        *c0 = tj * W::TS;
+    };
+    W rt[JR], lt[JL];
+#pragma unroll
+    for (int j = 0; j < JR; j++) {
+        const uint32_t k = kl + wave + j * NW;
+        uint32_t r0, c0;
+        origin(k, &r0, &c0);
+        rt[j].load(rs, r0, c0, R, C, lane, k < nk);
+    }
+#pragma unroll
+    for (int j = 0; j < JL; j++) {
+        const uint32_t k = wave + j * NW;
+        uint32_t r0, c0;
+        origin(k, &r0, &c0);
+        lt[j].load(rs, r0, c0, R, C, lane, k < kl);
+    }
+#pragma unroll
+    for (int j = 0; j < JL; j++) {
+        const uint32_t k = wave + j * NW;
+        if (k < kl) {
+            lt[j].transpose();
+#pragma unroll
+            for (uint32_t q = 0; q < W::PER; q++) lds[k * TV + q * 64 + lane] = lt[j].v[q];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < JR; j++) rt[j].transpose();
+    group_sync(bar, ctl, m, g);
+#pragma unroll
+    for (int j = 0; j < JL; j++) {
+        const uint32_t k = wave + j * NW;
+        if (k < kl) {
+            W x;
+#pragma unroll
+            for (uint32_t q = 0; q < W::PER; q++) x.v[q] = lds[k * TV + q * 64 + lane];
+            uint32_t r0, c0;
+            origin(k, &r0, &c0);
+            x.store_t(rs, r0, c0, R, C, lane, true);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < JR; j++) {
+        const uint32_t k = kl + wave + j * NW;
+        uint32_t r0, c0;
+        origin(k, &r0, &c0);
+        rt[j].store_t(rs, r0, c0, R, C, lane, k < nk);
+    }
+    if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
+}
+
+// Vector-path GCS kernel: slot s is member s % g of slice s / g.
+template <typename T, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_gcs2(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
+       uint32_t KL, Div32 dntj, uint32_t ntiles, uint32_t* __restrict__ ctl,
+       uint32_t* __restrict__ cnt) {
+    IPT_DYN_LDS(u32x4, lds);
+    const uint64_t total = nslices * g;
+    const uint64_t slice_elems = (uint64_t)R * C;
+    for (uint64_t slot = blockIdx.x; slot < total; slot += gridDim.x) {
+        const uint64_t S = slot / g;
+        const uint32_t m = (uint32_t)(slot - S * g);
+        gcs2_body<T, BLOCK>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
+                            g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds);
+        __syncthreads();  // LDS is reused by the next slot
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scalar path (a dimension not a multiple of 16 bytes): TS x TS tiles of
+// single elements staged in LDS with a padded pitch TS + 1, so a column read
+// hits distinct banks; the pad column is never read.
+template <typename T, int TS>
+struct GcsTile {
+    static constexpr uint32_t TP = TS + 1;     // padded LDS pitch
+    static constexpr uint32_t TILE = TS * TP;  // elements per staged tile
+    static constexpr uint32_t VPT = TS * TS;   // elements per tile
+};
+
+// Registers each thread holds across the barrier on the scalar path.
+constexpr int gcs_scalar_rv(int esz) { return esz == 4 ? 64 : 32; }
+
+// Element v of tile t (input layout): its offset, false when outside.
+template <int TS>
 __device__ __forceinline__ bool gcs_src(uint32_t t, uint32_t v, uint32_t R, uint32_t C, Div32 dntj,
                                         uint64_t* off) {
-    using G = GcsTile<T, TS, VEC>;
     const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
-    const uint32_t r = ti * TS + v / G::VPR, c = tj * TS + (v % G::VPR) * G::PER;
+    const uint32_t r = ti * TS + v / TS, c = tj * TS + v % TS;
     *off = (uint64_t)r * C + c;
     return r < R && c < C;
 }
 
-// Scatter one loaded vector into its staged tile (input layout, padded).
-template <typename T, int TS, bool VEC>
-__device__ __forceinline__ void gcs_put(T* tile, uint32_t v, const typename GcsTile<T, TS, VEC>::V& x) {
-    using G = GcsTile<T, TS, VEC>;
-    const T* e = reinterpret_cast<const T*>(&x);
-#pragma unroll
-    for (uint32_t q = 0; q < G::PER; q++) tile[(v / G::VPR) * G::TP + (v % G::VPR) * G::PER + q] = e[q];
+// LDS index of element v of staged tile slot k (input layout, padded).
+template <int TS>
+__device__ __forceinline__ uint32_t gcs_slot(uint32_t k, uint32_t v) {
+    return k * GcsTile<uint32_t, TS>::TILE + (v / TS) * (TS + 1) + v % TS;
 }
 
 // Write staged tiles [k0, k0 + n) from LDS slots [0, n) transposed: output
-// row = input column.  Four vectors per thread are read from LDS before the
-// first store is issued.
-template <typename T, int TS, bool VEC, int BLOCK>
+// row = input column.  Four elements per thread are read from LDS before
+// the first store is issued.
+template <typename T, int TS, int BLOCK>
 __device__ __forceinline__ void gcs_store(T* __restrict__ base, const T* lds, uint32_t R, uint32_t C,
                                           Div32 dntj, uint32_t m, uint32_t g, uint32_t k0,
                                           uint32_t n) {
-    using G = GcsTile<T, TS, VEC>;
-    using V = typename G::V;
+    using G = GcsTile<T, TS>;
     constexpr uint32_t W = 4;
     const uint32_t total = n * G::VPT;
     for (uint32_t i0 = 0; i0 < total; i0 += W * BLOCK) {
-        union { V v; T e[G::PER]; } u[W];
+        T u[W];
         uint64_t off[W];
         bool ok[W];
 #pragma unroll
@@ -422,88 +620,83 @@ __device__ __forceinline__ void gcs_store(T* __restrict__ base, const T* lds, ui
                 const uint32_t k = idx / G::VPT, x = idx % G::VPT;
                 const uint32_t t = m + (k0 + k) * g;
                 const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
-                const uint32_t cc = x / G::VPR, rr = (x % G::VPR) * G::PER;  // output row cc, cols rr..
+                const uint32_t cc = x / TS, rr = x % TS;  // output row cc, column rr
                 const uint32_t orow = tj * TS + cc, ocol = ti * TS + rr;
                 ok[w] = orow < C && ocol < R;
                 off[w] = (uint64_t)orow * R + ocol;
-                const T* tile = lds + k * G::TILE;
-#pragma unroll
-                for (uint32_t q = 0; q < G::PER; q++) u[w].e[q] = tile[(rr + q) * G::TP + cc];
+                u[w] = lds[k * G::TILE + rr * G::TP + cc];
             }
         }
 #pragma unroll
         for (uint32_t w = 0; w < W; w++)
-            if (ok[w]) *reinterpret_cast<V*>(base + off[w]) = u[w].v;
+            if (ok[w]) base[off[w]] = u[w];
     }
 }
 
-// One GCS slot: member m of the group of g that transposes the slice at
-// `base`.  Member m owns tiles t = m + k*g, k < nk; the first kl of them are
-// staged in LDS, the remaining nk - kl (at most RV * BLOCK / VPT) stay in
-// registers across the group barrier and go through LDS after the
-// LDS-staged tiles have been written.  `bar` is the slice's barrier state.
-template <typename T, int TS, bool VEC, int BLOCK, int RV>
+// One scalar-path GCS slot: member m of the group of g that transposes the
+// slice at `base`.  Member m owns tiles t = m + k*g, k < nk; the first kl of
+// them are staged in LDS, the remaining nk - kl (at most RV * BLOCK / VPT)
+// stay in registers across the group barrier and go through LDS after the
+// LDS-staged tiles have been written.  Loads and LDS writes are free of
+// branches (a lane with nothing to do loads element 0 and writes the pad
+// element of slot 0, which is never read), so the compiler keeps every
+// load of a batch in flight.
+template <typename T, int TS, int BLOCK, int RV>
 __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
                                          uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
                                          uint32_t m, uint32_t* bar, uint32_t* ctl, T* lds) {
-    using G = GcsTile<T, TS, VEC>;
-    using V = typename G::V;
-    constexpr uint32_t U = 32;  // LDS-bound vectors in flight per thread
+    using G = GcsTile<T, TS>;
+    constexpr uint32_t U = 32;  // LDS-bound elements in flight per thread
+    constexpr uint32_t PAD = TS;  // slot 0, row 0, pad column
     const uint32_t nk = m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0;
     const uint32_t kl = min(nk, KL), kr = nk - kl;
 
-    // register-held tiles: issued first, kept across the barrier
-    V rreg[RV];
+    T rreg[RV];
     const uint32_t rtotal = kr * G::VPT;
+    if (kr) {
 #pragma unroll
-    for (uint32_t u = 0; u < RV; u++) {
-        const uint32_t idx = u * BLOCK + threadIdx.x;
-        if (idx < rtotal) {
+        for (uint32_t u = 0; u < RV; u++) {
+            const uint32_t idx = u * BLOCK + threadIdx.x;
             uint64_t off;
-            if (gcs_src<T, TS, VEC>(m + (kl + idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off))
-                rreg[u] = *reinterpret_cast<const V*>(base + off);
+            const bool ok = idx < rtotal &&
+                            gcs_src<TS>(m + (kl + idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off);
+            rreg[u] = base[ok ? off : 0];
         }
     }
-    // LDS-staged tiles, U vectors per thread in flight per batch
     const uint32_t ltotal = kl * G::VPT;
     for (uint32_t b0 = 0; b0 < ltotal; b0 += U * BLOCK) {
-        V reg[U];
+        T reg[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t idx = b0 + u * BLOCK + threadIdx.x;
-            if (idx < ltotal) {
-                uint64_t off;
-                if (gcs_src<T, TS, VEC>(m + (idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off))
-                    reg[u] = *reinterpret_cast<const V*>(base + off);
-            }
+            uint64_t off;
+            const bool ok = idx < ltotal &&
+                            gcs_src<TS>(m + (idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off);
+            reg[u] = base[ok ? off : 0];
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t idx = b0 + u * BLOCK + threadIdx.x;
-            if (idx < ltotal) gcs_put<T, TS, VEC>(lds + (idx / G::VPT) * G::TILE, idx % G::VPT, reg[u]);
+            lds[idx < ltotal ? gcs_slot<TS>(idx / G::VPT, idx % G::VPT) : PAD] = reg[u];
         }
     }
-    __syncthreads();
-    if (g > 1) {
-        if (threadIdx.x == 0 && !group_arrive_wait(bar, m, g)) atomic_store_agent(&ctl[2], 1u);
-        __syncthreads();
-    }
-    gcs_store<T, TS, VEC, BLOCK>(base, lds, R, C, dntj, m, g, 0, kl);
+    group_sync(bar, ctl, m, g);
+    gcs_store<T, TS, BLOCK>(base, lds, R, C, dntj, m, g, 0, kl);
     if (kr) {
         __syncthreads();
 #pragma unroll
         for (uint32_t u = 0; u < RV; u++) {
             const uint32_t idx = u * BLOCK + threadIdx.x;
-            if (idx < rtotal) gcs_put<T, TS, VEC>(lds + (idx / G::VPT) * G::TILE, idx % G::VPT, rreg[u]);
+            lds[idx < rtotal ? gcs_slot<TS>(idx / G::VPT, idx % G::VPT) : PAD] = rreg[u];
         }
         __syncthreads();
-        gcs_store<T, TS, VEC, BLOCK>(base, lds, R, C, dntj, m, g, kl, kr);
+        gcs_store<T, TS, BLOCK>(base, lds, R, C, dntj, m, g, kl, kr);
     }
     if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
 }
 
-// GCS kernel: slot s is member s % g of slice s / g.
-template <typename T, int TS, bool VEC, int BLOCK, int RV>
+// Scalar-path GCS kernel: slot s is member s % g of slice s / g.
+template <typename T, int TS, int BLOCK, int RV>
 __global__ void __launch_bounds__(BLOCK)
 k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
       uint32_t KL, Div32 dntj, uint32_t ntiles, uint32_t* __restrict__ ctl,
@@ -514,8 +707,8 @@ k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g
     for (uint64_t slot = blockIdx.x; slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
-        gcs_body<T, TS, VEC, BLOCK, RV>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
-                                        g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds);
+        gcs_body<T, TS, BLOCK, RV>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
+                                   g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds);
         __syncthreads();  // LDS is reused by the next slot
     }
 }
@@ -529,7 +722,8 @@ namespace ipt {
 // items.  Each instance (d3 slices of R x C, 4- or 8-byte elements) is cut
 // into items of one kind:
 //   BK_SLICE  `sub` consecutive slices staged whole in LDS (slice_body)
-//   BK_GCS    member `sub` of the group of g that owns one slice (gcs_body);
+//   BK_GCS    member `sub` of the group of g that owns one slice (gcs2_body
+//             on the vector path, gcs_body on the scalar path);
 //             the g members of a slice occupy consecutive items
 //   BK_SWAP   tile pair `sub` of one square slice (swap_body)
 // Workgroup b takes items b, b + grid, ... (static round-robin), so the
@@ -567,14 +761,14 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
         slice_body<T>(base, I.R, I.C, I.P, it.sub, I.dS, I.dC, I.dR, lds);
         break;
     case BK_GCS_VEC:
-        gcs_body<T, TS_GCS, true, kBatchBlock, 8>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles,
-                                                  it.sub, bars + I.bar + bar_words(I.g) * it.slice,
-                                                  ctl, lds);
+        gcs2_body<T, kBatchBlock>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles, it.sub,
+                                  bars + I.bar + bar_words(I.g) * it.slice, ctl,
+                                  reinterpret_cast<u32x4*>(lds));
         break;
     case BK_GCS_SCALAR:
-        gcs_body<T, TS_GCS, false, kBatchBlock, 64>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
-                                                   I.ntiles, it.sub,
-                                                   bars + I.bar + bar_words(I.g) * it.slice, ctl, lds);
+        gcs_body<T, TS_GCS, kBatchBlock, gcs_scalar_rv(sizeof(T))>(
+            base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles, it.sub,
+            bars + I.bar + bar_words(I.g) * it.slice, ctl, lds);
         break;
     case BK_SWAP_VEC:
         swap_body<T, TS_SWAP, kBatchBlock, true>(base, I.R, it.sub, lds);
