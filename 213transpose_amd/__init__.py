@@ -24,11 +24,12 @@ import types
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libinplace_hip.so")
 
-VARIANT_AUTO, VARIANT_SLICE_LDS, VARIANT_SQUARE_SWAP, VARIANT_C2R, VARIANT_R2C, VARIANT_GCS = range(6)
+VARIANT_AUTO, VARIANT_SLICE_LDS, VARIANT_SQUARE_SWAP, VARIANT_C2R, VARIANT_R2C, VARIANT_GCS, \
+    VARIANT_SKINNY = range(7)
 VARIANT_NAMES = {VARIANT_AUTO: "auto", VARIANT_SLICE_LDS: "slice_lds",
                  VARIANT_SQUARE_SWAP: "square_swap", VARIANT_C2R: "c2r", VARIANT_R2C: "r2c",
-                 VARIANT_GCS: "gcs"}
-N_VARIANTS = 6
+                 VARIANT_GCS: "gcs", VARIANT_SKINNY: "skinny"}
+N_VARIANTS = 7
 
 # Map ids of ipt_debug_eval_map.
 MAP_C2R_PRE, MAP_C2R_POST, MAP_R2C_PRE, MAP_R2C_POST, MAP_C2R_SHUFFLE, MAP_R2C_SHUFFLE = range(6)

@@ -158,6 +158,34 @@ ipt_status validate(const void* data, int esz, int64_t d1, int64_t d2, int64_t d
     return IPT_OK;
 }
 
+// IPT_VARIANT_AUTO: the learned selector's candidate when a selector is
+// loaded and no tune is forced, else the heuristic's variant.  Where the
+// skinny path applies it replaces a decomposition choice (it costs about
+// two sweeps where C2R / R2C cost three or more); when it cannot run
+// (unaligned data, more saved chunks than the scratch cap, or stream capture
+// before the shape's first use) the decomposition runs instead.
+ipt_status run_auto(DeviceCtx* ctx, void* data, const Shape& s, const ipt_tune* tune,
+                    hipStream_t st) {
+    int v;
+    Tune t;
+    if (!tune && selector_loaded()) {
+        const Candidate& c = kCandidates[select_candidate(ctx, s)];
+        v = c.variant;
+        t = resolve_tune(ctx, &c.tune);
+    } else {
+        v = choose_variant(ctx, s);
+        t = resolve_tune(ctx, tune);
+    }
+    if (v == IPT_VARIANT_C2R || v == IPT_VARIANT_R2C || v == IPT_VARIANT_SKINNY) {
+        if (skinny_plan(s).ok) {
+            const ipt_status r = run_variant(ctx, data, s, IPT_VARIANT_SKINNY, resolve_tune(ctx, tune), st);
+            if (r != IPT_ERR_UNSUPPORTED) return r;
+        }
+        if (v == IPT_VARIANT_SKINNY) v = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
+    }
+    return run_variant(ctx, data, s, v, t, st);
+}
+
 }  // namespace ipt
 
 using namespace ipt;
@@ -191,13 +219,8 @@ ipt_status ipt_transpose_variant(void* data, int elem_bytes, int64_t d1, int64_t
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
     Shape s = make_shape(elem_bytes, d1, d2, d3);
-    if (variant == IPT_VARIANT_AUTO && !tune && selector_loaded()) {
-        const Candidate& c = kCandidates[select_candidate(ctx, s)];
-        return run_variant(ctx, data, s, c.variant, resolve_tune(ctx, &c.tune), (hipStream_t)stream);
-    }
-    Tune t = resolve_tune(ctx, tune);
-    if (variant == IPT_VARIANT_AUTO) variant = choose_variant(ctx, s);
-    return run_variant(ctx, data, s, variant, t, (hipStream_t)stream);
+    if (variant == IPT_VARIANT_AUTO) return run_auto(ctx, data, s, tune, (hipStream_t)stream);
+    return run_variant(ctx, data, s, variant, resolve_tune(ctx, tune), (hipStream_t)stream);
 }
 
 ipt_status ipt_transpose_213(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,

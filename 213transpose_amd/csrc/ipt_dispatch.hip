@@ -33,24 +33,6 @@ k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
 }
 
 
-#define IPT_LAUNCH_CHECK(what)                          \
-    do {                                                \
-        hipError_t e_ = hipGetLastError();              \
-        if (e_ != hipSuccess) {                         \
-            set_error(what, e_);                        \
-            return IPT_ERR_HIP;                         \
-        }                                               \
-    } while (0)
-
-#define IPT_HIP_CHECK(call)                             \
-    do {                                                \
-        hipError_t e_ = (call);                         \
-        if (e_ != hipSuccess) {                         \
-            set_error(#call, e_);                       \
-            return IPT_ERR_HIP;                         \
-        }                                               \
-    } while (0)
-
 int64_t lds_max_of(const DeviceCtx* ctx) { return ctx ? ctx->lds_max : kDefaultLds; }
 int ncu_of(const DeviceCtx* ctx) { return ctx ? ctx->ncu : 256; }
 
@@ -133,6 +115,7 @@ bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant) {
     case IPT_VARIANT_C2R:
     case IPT_VARIANT_R2C: return true;
     case IPT_VARIANT_GCS: return gcs_geom(ctx, s, lds_max_of(ctx)).ok;
+    case IPT_VARIANT_SKINNY: return skinny_plan(s).ok;
     default: return false;
     }
 }
@@ -147,6 +130,7 @@ int choose_variant(const DeviceCtx* ctx, const Shape& s) {
     if (gcs_geom(ctx, s, lds_max_of(ctx)).ok) return IPT_VARIANT_GCS;
     if (slice_lds_fits(ctx, s)) return IPT_VARIANT_SLICE_LDS;
     if (s.d1 == s.d2) return IPT_VARIANT_SQUARE_SWAP;
+    if (skinny_plan(s).ok) return IPT_VARIANT_SKINNY;
     return s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
 }
 
@@ -220,13 +204,17 @@ static ipt_status run_slice_lds(DeviceCtx* ctx, T* data, const Shape& s, const T
     uint32_t per = (uint32_t)std::max<int64_t>(1, budget / slice_bytes);
     per = (uint32_t)std::min<uint64_t>(per, (uint64_t)s.d3);
     const size_t lds = (size_t)per * slice_bytes;
-    auto k = k_slice_lds<T>;
+    // Vector form: every slice a multiple of 16 bytes at a 16-byte aligned
+    // base; it runs 256-thread workgroups (its compile-time block).
+    const bool vec = ((uint64_t)R * C * sizeof(T)) % 16 == 0 && (uintptr_t)data % 16 == 0;
+    auto k = vec ? k_slice_lds<T, true> : k_slice_lds<T, false>;
+    const int block = vec ? 256 : t.block;
     ipt_status stt = allow_lds(ctx, k, lds);
     if (stt != IPT_OK) return stt;
     const uint64_t items = ((uint64_t)s.d3 + per - 1) / per;
     int grid;
-    grid_for(ctx, k, t.block, lds, items, t.grid_mult, &grid);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(t.block), lds, st, data, R, C, P, (uint64_t)s.d3, per,
+    grid_for(ctx, k, block, lds, items, t.grid_mult, &grid);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, st, data, R, C, P, (uint64_t)s.d3, per,
                        Div32::make(R * C), Div32::make(C), Div32::make(R));
     IPT_LAUNCH_CHECK("k_slice_lds");
     return IPT_OK;
@@ -453,6 +441,7 @@ static ipt_status run_typed(DeviceCtx* ctx, T* data, const Shape& s, int variant
     case IPT_VARIANT_C2R: return run_decomp<T>(ctx, data, s, true, t, st);
     case IPT_VARIANT_R2C: return run_decomp<T>(ctx, data, s, false, t, st);
     case IPT_VARIANT_GCS: return run_gcs<T>(ctx, data, s, t, st);
+    case IPT_VARIANT_SKINNY: return run_skinny(ctx, data, s, t, st);
     default: return IPT_ERR_INVALID_ARG;
     }
 }

@@ -56,6 +56,26 @@ constexpr int64_t kDefaultLds = 160 * 1024;       // gfx950 LDS per workgroup
 constexpr size_t kScratchCap = (size_t)256 << 20;  // fallback-tier scratch bound
 
 void set_error(const char* what, hipError_t e);
+
+// Return IPT_ERR_HIP (recording the call) when a launch or HIP call failed.
+#define IPT_LAUNCH_CHECK(what)                          \
+    do {                                                \
+        hipError_t e_ = hipGetLastError();              \
+        if (e_ != hipSuccess) {                         \
+            set_error(what, e_);                        \
+            return IPT_ERR_HIP;                         \
+        }                                               \
+    } while (0)
+
+#define IPT_HIP_CHECK(call)                             \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) {                         \
+            set_error(#call, e_);                       \
+            return IPT_ERR_HIP;                         \
+        }                                               \
+    } while (0)
+
 ipt_status get_ctx(DeviceCtx** out);
 ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr);
 ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev);
@@ -76,6 +96,8 @@ int tuned_candidate(const Shape& s);  // -1 when the shape has no measured choic
 bool candidate_ok(const DeviceCtx* ctx, const Shape& s, int k);
 ipt_status run_variant(DeviceCtx* ctx, void* data, const Shape& s, int variant, const Tune& t,
                        hipStream_t stream);
+ipt_status run_auto(DeviceCtx* ctx, void* data, const Shape& s, const ipt_tune* tune,
+                    hipStream_t stream);
 ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st);
 ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, int64_t d3,
                     int map_id, const Tune& t, hipStream_t stream);
@@ -104,6 +126,24 @@ int64_t slice_pitch(int64_t C);
 int64_t lds_max_of(const DeviceCtx* ctx);
 int ncu_of(const DeviceCtx* ctx);
 bool decomp_needs_scratch(const Shape& s, bool c2r, const Tune& t);
+
+// Skinny path (ipt_skinny.hip): one short side s, long side L = M * B.
+constexpr int64_t kSkinnyMaxShort = 128;      // largest short side
+constexpr int64_t kSkinnyBlockBytes = 64 * 1024;  // s x B block, one LDS tile
+constexpr int64_t kSkinnyMinChunk = 512;      // bytes per moved chunk
+constexpr int64_t kSkinnyStageBytes = 64 * 1024;  // chunks staged per segment
+constexpr uint32_t kSkinnyMaxSeg = 128;       // positions per segment
+
+struct SkinnyPlan {
+    bool ok = false;
+    bool tall = false;   // R > C (s = C); else s = R
+    int64_t s = 0, L = 0, B = 0, M = 0;
+    int64_t rc = 0, cc = 0;  // chunk matrix transposed by the chunk step
+    int64_t cv = 0;          // 16-byte vectors per chunk
+    int64_t ell = 0;         // chunks per segment (at most)
+};
+SkinnyPlan skinny_plan(const Shape& s);
+ipt_status run_skinny(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t, hipStream_t st);
 
 template <typename Ix>
 ipt_status eval_map_host(int map_id, const HostPlan& h, const int64_t* i, const int64_t* j,

@@ -79,15 +79,71 @@ __device__ __forceinline__ void slice_body(T* __restrict__ base, uint32_t R, uin
     }
 }
 
-// Each work item is `per` consecutive slices.
-template <typename T>
+// Vector form of slice_body for slices of a multiple of 16 bytes at a
+// 16-byte aligned base: 16-byte loads (U per thread in flight; lanes past
+// the end repeat the last vector, so the loads carry no branches), scattered
+// into the odd-pitch LDS image with one division per vector, then 16-byte
+// stores assembled from a column walk of the image.  Row r of the staged
+// slices (all ns of them stacked) starts at LDS r * P.
+template <typename T, int BLOCK>
+__device__ __forceinline__ void slice_body_vec(T* __restrict__ base, uint32_t R, uint32_t C,
+                                               uint32_t P, uint32_t ns, Div32 dS, Div32 dC,
+                                               Div32 dR, T* lds) {
+    constexpr uint32_t PER = 16 / sizeof(T);
+    constexpr uint32_t U = 8;
+    using V = union { uint4 v; T e[PER]; };
+    const uint32_t S = R * C;
+    const uint32_t nv = ns * S / PER;
+    const uint4* src = reinterpret_cast<const uint4*>(base);
+    for (uint32_t v0 = 0; v0 < nv; v0 += U * BLOCK) {
+        V x[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) x[u].v = src[min(v0 + u * BLOCK + threadIdx.x, nv - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t e = min(v0 + u * BLOCK + threadIdx.x, nv - 1) * PER;
+            uint32_t r = dC.div(e), j = e - r * C;  // stacked row, column
+#pragma unroll
+            for (uint32_t q = 0; q < PER; q++) {
+                lds[r * P + j] = x[u].e[q];
+                const bool wrap = ++j == C;
+                j = wrap ? 0 : j;
+                r += wrap ? 1 : 0;
+            }
+        }
+    }
+    __syncthreads();
+    uint4* dst = reinterpret_cast<uint4*>(base);
+    for (uint32_t v = threadIdx.x; v < nv; v += BLOCK) {
+        const uint32_t o = v * PER;
+        const uint32_t sl = dS.div(o), rem = o - sl * S;
+        uint32_t jj = dR.div(rem), ii = rem - jj * R;  // output row jj (input column), col ii
+        uint32_t row0 = sl * R;                         // stacked row of input row 0 of slice sl
+        V y;
+#pragma unroll
+        for (uint32_t q = 0; q < PER; q++) {
+            y.e[q] = lds[(row0 + ii) * P + jj];
+            const bool wrap = ++ii == R;
+            ii = wrap ? 0 : ii;
+            const bool next = wrap && ++jj == C;
+            jj = next ? 0 : jj;
+            row0 += next ? R : 0;
+        }
+        dst[v] = y.v;
+    }
+}
+
+// Each work item is `per` consecutive slices.  VEC: slices of a multiple of
+// 16 bytes at a 16-byte aligned base (slice_body_vec).
+template <typename T, bool VEC>
 __global__ void __launch_bounds__(1024)
 k_slice_lds(T* __restrict__ data, uint32_t R, uint32_t C, uint32_t P, uint64_t nslices,
             uint32_t per, Div32 dS, Div32 dC, Div32 dR) {
     IPT_DYN_LDS(T, lds);
     for (uint64_t s0 = (uint64_t)blockIdx.x * per; s0 < nslices; s0 += (uint64_t)gridDim.x * per) {
         const uint32_t ns = (uint32_t)umin64(per, nslices - s0);
-        slice_body(data + s0 * R * C, R, C, P, ns, dS, dC, dR, lds);
+        if (VEC) slice_body_vec<T, 256>(data + s0 * R * C, R, C, P, ns, dS, dC, dR, lds);
+        else slice_body(data + s0 * R * C, R, C, P, ns, dS, dC, dR, lds);
         __syncthreads();
     }
 }

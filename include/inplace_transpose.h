@@ -92,7 +92,13 @@ typedef enum {
                                     stages a slice in LDS, then writes it
                                     transposed; one sweep for slices up to the
                                     GPU's resident LDS */
-    IPT_VARIANT_COUNT = 6
+    IPT_VARIANT_SKINNY = 6,      /* one short side s (<= 128): blocks of B rows
+                                    (or columns) of the long side transposed
+                                    in LDS, and the B-element chunks moved by
+                                    segmented cycle following (the skinny path
+                                    of src/skinny.cu:313-363, rebuilt); about
+                                    two sweeps */
+    IPT_VARIANT_COUNT = 7
 } ipt_variant;
 
 /* Tuning knobs; zero fields mean "library default". */
