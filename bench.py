@@ -27,8 +27,10 @@ The JSON line also carries:
                 two tensors, on this host's cores, bounded to ~10 s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-       python bench.py --profile ...   (workload + calibration copies only,
-                                        for running under a profiler)
+       python bench.py --profile [--cands K2A,K2B] ...
+                                       (calibration copies, then each workload
+                                        kernel alone, for running under a
+                                        profiler)
 """
 import argparse
 import glob
@@ -64,6 +66,8 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE child runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--cands", default=None,
+                    help="--profile: selector candidate per workload tensor, comma-separated")
     ap.add_argument("--profile", action="store_true",
                     help="only run workload steps and the calibration copies (for profilers)")
     return ap.parse_args()
@@ -155,15 +159,30 @@ def make_tensors(torch):
 
 
 def profile_main(args, torch, ipt):
-    """Workload steps + calibration copies, nothing else (runs under rocprofv3)."""
+    """The per-kernel measurement's sequence and nothing else (runs under
+    rocprofv3): calibration copies of both tensors first, then each workload
+    tensor transposed back to back on its own, as the per-kernel timing does,
+    so every kernel sees the cache residency it had when it was timed.
+    --cands picks the selector candidate per tensor (the autotuned choice the
+    timed run used); without it the library's dispatch decides."""
     tensors = make_tensors(torch)
     sinks = [torch.empty_like(t) for t in tensors]
     for _ in range(args.warmup + args.steps):
-        for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
-            ipt.transpose(t, d1, d2, d3)
         for t, s in zip(tensors, sinks):
             ipt.copy(s, t)
     torch.cuda.synchronize()
+    del sinks
+    torch.cuda.empty_cache()
+    cands = [int(x) for x in args.cands.split(",")] if args.cands else [None] * len(tensors)
+    for t, (d1, d2, d3, esz), k in zip(tensors, WORKLOAD, cands):
+        if k is None:
+            fn = lambda: ipt.transpose(t, d1, d2, d3)
+        else:
+            v, tune = ipt.candidate(k)
+            fn = lambda: ipt.transpose_variant(t, d1, d2, d3, v, tune=tune)
+        for _ in range(args.warmup + args.steps):
+            fn()
+        torch.cuda.synchronize()
     print("profile run done", flush=True)
 
 
@@ -178,7 +197,7 @@ def pmc_per_dispatch(db, counter):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def measure_traffic(match, shape_esz):
+def measure_traffic(match, shape_esz, cands):
     """Bytes per launch of the kernel whose name contains `match`, from
     FETCH_SIZE and WRITE_SIZE, each scaled per byte by the copy kernel that
     copies the same tensor (same bytes read and written per dispatch)."""
@@ -190,7 +209,7 @@ def measure_traffic(match, shape_esz):
         out = tempfile.mkdtemp(prefix="ipt_pmc_")
         cmd = ["timeout", "-s", "KILL", "120", "rocprofv3", "--pmc", counter, "-d", out, "-o",
                "pmc", "--", sys.executable, os.path.abspath(__file__), "--profile", "--steps", "3",
-               "--warmup", "1"]
+               "--warmup", "1", "--cands", cands]
         r = subprocess.run(cmd, cwd=tempfile.gettempdir(), capture_output=True, text=True)
         dbs = glob.glob(os.path.join(out, "**", "*.db"), recursive=True)
         if r.returncode not in (0, 139) or not dbs:
@@ -355,7 +374,8 @@ def main():
 
     traffic, traffic_note = None, "not measured (--no-traffic)"
     if not args.no_traffic:
-        totals, err = measure_traffic(dom["match"], tuple(dom["shape"]))
+        cands = ",".join(str(tuned[f"{d1}x{d2}x{d3}x{esz}"]["best"]) for d1, d2, d3, esz in WORKLOAD)
+        totals, err = measure_traffic(dom["match"], tuple(dom["shape"]), cands)
         if totals:
             traffic = int(totals["FETCH_SIZE"] + totals["WRITE_SIZE"])
             traffic_note = (f"FETCH {totals['FETCH_SIZE'] / 1e6:.1f} MB + WRITE "
