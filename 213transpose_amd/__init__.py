@@ -25,10 +25,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libinplace_hip.so")
 
 VARIANT_AUTO, VARIANT_SLICE_LDS, VARIANT_SQUARE_SWAP, VARIANT_C2R, VARIANT_R2C, VARIANT_GCS, \
-    VARIANT_SKINNY = range(7)
+    VARIANT_BLOCK_CYCLE = range(7)
 VARIANT_NAMES = {VARIANT_AUTO: "auto", VARIANT_SLICE_LDS: "slice_lds",
                  VARIANT_SQUARE_SWAP: "square_swap", VARIANT_C2R: "c2r", VARIANT_R2C: "r2c",
-                 VARIANT_GCS: "gcs", VARIANT_SKINNY: "skinny"}
+                 VARIANT_GCS: "gcs", VARIANT_BLOCK_CYCLE: "block_cycle"}
 N_VARIANTS = 7
 
 # Map ids of ipt_debug_eval_map.

@@ -206,10 +206,10 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         b->info.bytes += (uint64_t)s.d1 * s.d2 * s.d3 * s.esz;
         if (p == Route::Big) {
             if (big_variant == IPT_VARIANT_AUTO) big_variant = choose_variant(ctx, s);
-            // The skinny path allocates its segment table on a shape's first
-            // use and stages chunks in scratch, neither of which a captured
-            // batch may do: its instances take the decomposition.
-            if (big_variant == IPT_VARIANT_SKINNY)
+            // The block-cycle path allocates its segment table on a shape's
+            // first use and stages chunks in scratch, neither of which a
+            // captured batch may do: its instances take the decomposition.
+            if (big_variant == IPT_VARIANT_BLOCK_CYCLE)
                 big_variant = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
             const bool c2r = big_variant == IPT_VARIANT_C2R;
             if ((big_variant == IPT_VARIANT_C2R || big_variant == IPT_VARIANT_R2C) &&

@@ -1,0 +1,368 @@
+// ipt_block_cycle.hip — the block-cycle path (IPT_VARIANT_BLOCK_CYCLE).
+//
+// One side of an R x C slice (the chunked side, length L = nb * T) is cut
+// into chunks of T elements; the slice is then transposed in two steps:
+//   chunk step  the chunks move as a matrix of chunks is transposed, by
+//               segmented cycle following (the reference's cycle-following
+//               permute, src/permute.cu:18-137, rebuilt on whole chunks);
+//   block step  nb blocks of other x T (or T x other) elements, each a
+//               contiguous slice, are transposed in place by a one-sweep
+//               kernel: the square swap, the cooperative kernel (GCS) or the
+//               whole-slice kernel.
+// Chunking the columns (C = nb * T) runs the chunk step first, chunking the
+// rows (R = nb * T) runs the block step first.  With a short side s <= 128
+// and blocks that fit LDS this is the skinny path (src/skinny.cu:313-363,
+// rebuilt); with larger blocks it covers big rectangular slices that fit
+// neither the square swap nor one cooperative group.  Cycles are cut into
+// segments on the host once per chunk-matrix shape (cached per device);
+// every segment's first chunk is saved to scratch, and then each segment is
+// moved by one workgroup with no synchronisation between segments.  Both
+// steps stream contiguous 16-byte vectors: about two sweeps.
+#include "ipt_internal.h"
+#include "ipt_kernels.h"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <set>
+#include <tuple>
+#include <vector>
+
+namespace ipt {
+
+// ---------------------------------------------------------------------------
+// Chunk permutation.  A slice is a sequence of N = rc * cc chunks of cv
+// 16-byte vectors, and the content of chunk q moves to
+//     P(q) = (q mod cc) * rc + q / cc
+// (the transpose of an rc x cc matrix of chunks).  The host walks the cycles
+// of P once per shape and writes a move plan: work item i is moves
+// [m0, m0 + nm) of (source, destination) chunk positions, nm <= ell.
+//   - A cycle of at most ell positions is moved whole by one item (several
+//     short cycles share an item); all its chunks are staged before any is
+//     written, so it needs nothing else.
+//   - A longer cycle is cut into segments of ell positions, one item each:
+//     the segment's first chunk was saved beforehand by k_chunk_save (its
+//     source is head slot `s`, flagged kHeadFlag; it is always move 0), the
+//     item stages the rest, and writes every chunk to the next position of
+//     the cycle, the last one to the first position of the next segment,
+//     whose content was saved.
+// Fixed points have no moves.  An item reads only positions it also writes,
+// and no position is written by two items, so once the heads are saved (a
+// kernel boundary) items need no synchronisation at all.
+constexpr int kChunkBlock = 256;
+constexpr uint32_t kHeadFlag = 0x80000000u;
+
+// head[(slice * nheads + h) * cv + v] = vector v of chunk hpos[h] of the
+// slice.  Lanes past the end repeat the last vector (same address, value).
+__global__ void __launch_bounds__(kChunkBlock)
+k_chunk_save(const uint4* __restrict__ data, uint4* __restrict__ head, const uint32_t* __restrict__ hpos,
+             uint32_t nheads, uint64_t nslices, uint64_t slice_vecs, uint32_t cv, Div32 dcv) {
+    const uint64_t per_slice = (uint64_t)nheads * cv;
+    const uint64_t total = nslices * per_slice;
+    constexpr uint32_t U = 4;
+    for (uint64_t e0 = (uint64_t)blockIdx.x * kChunkBlock * U; e0 < total;
+         e0 += (uint64_t)gridDim.x * kChunkBlock * U) {
+        uint4 r[U];
+        uint64_t dst[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint64_t e = umin64(e0 + u * kChunkBlock + threadIdx.x, total - 1);
+            const uint64_t sl = e / per_slice;
+            const uint32_t x = (uint32_t)(e - sl * per_slice);
+            const uint32_t h = dcv.div(x), v = x - h * cv;
+            dst[u] = e;
+            r[u] = data[sl * slice_vecs + (uint64_t)hpos[h] * cv + v];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) head[dst[u]] = r[u];
+    }
+}
+
+// One workgroup per (slice, item): the item's moves are read into LDS, its
+// source chunks staged in LDS (the saved head from `head`, the rest from the
+// slice), then each staged chunk written to its destination.  Staging lanes
+// past the end repeat the last vector, so the loads carry no branches and a
+// batch of U of them is in flight at once.
+__global__ void __launch_bounds__(kChunkBlock)
+k_chunk_cycle(uint4* __restrict__ data, const uint4* __restrict__ head, const uint2* __restrict__ items,
+              const uint32_t* __restrict__ msrc, const uint32_t* __restrict__ mdst, uint32_t nitems,
+              uint32_t nheads, uint64_t nslices, uint64_t slice_vecs, uint32_t cv, Div32 dcv) {
+    extern __shared__ __align__(16) uint4 stage[];
+    __shared__ uint32_t src[kBcMaxSeg], dst[kBcMaxSeg];
+    constexpr uint32_t U = 8;
+    const uint64_t total_items = nslices * nitems;
+    for (uint64_t it = blockIdx.x; it < total_items; it += gridDim.x) {
+        const uint64_t sl = it / nitems;
+        const uint2 item = items[it - sl * nitems];
+        const uint32_t nm = item.y;
+        uint4* base = data + sl * slice_vecs;
+        for (uint32_t j = threadIdx.x; j < nm; j += kChunkBlock) {
+            src[j] = msrc[item.x + j];
+            dst[j] = mdst[item.x + j];
+        }
+        __syncthreads();
+        uint32_t first = 0;
+        if (src[0] & kHeadFlag) {  // a cut cycle's segment: its head was saved
+            const uint4* h = head + (sl * nheads + (src[0] & ~kHeadFlag)) * cv;
+            for (uint32_t v = threadIdx.x; v < cv; v += kChunkBlock) stage[v] = h[v];
+            first = 1;
+        }
+        const uint32_t total = (nm - first) * cv;
+        for (uint32_t e0 = 0; e0 < total; e0 += U * kChunkBlock) {
+            uint4 r[U];
+            uint32_t at[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t e = min(e0 + u * kChunkBlock + threadIdx.x, total - 1);
+                const uint32_t j = dcv.div(e), v = e - j * cv;
+                at[u] = first * cv + e;
+                r[u] = base[(uint64_t)src[first + j] * cv + v];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) stage[at[u]] = r[u];
+        }
+        __syncthreads();
+        const uint32_t wtotal = nm * cv;
+        for (uint32_t e = threadIdx.x; e < wtotal; e += kChunkBlock) {
+            const uint32_t j = dcv.div(e), v = e - j * cv;
+            base[(uint64_t)dst[j] * cv + v] = stage[e];
+        }
+        __syncthreads();  // stage, src and dst are reused by the next item
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plan.  Candidates: both orientations, every divisor T of the chunked side
+// with T <= L / 2 (at least two blocks), a chunk of kBcMinChunk ..
+// kBcMaxChunk bytes (a multiple of 16), fewer than kBcMaxChunks per slice and
+// a block some one-sweep kernel runs.  Each is priced in sweeps:
+//   chunk step  (1 + 1.5 / ell): the saved heads are read and written once
+//               and read again, 1/ell of the data; chunks under 2 KB cost
+//               extra (short contiguous runs per position);
+//   block step  1 / efficiency of its kernel relative to a copy (measured:
+//               square swap ~1, whole-slice 16-byte form ~0.9, cooperative
+//               vector path ~0.85, cooperative scalar path ~0.5 or less).
+// The cheapest wins; ties go to the larger chunk.
+static int block_kernel(const DeviceCtx* ctx, const Shape& blk, double* eff) {
+    const int64_t bytes = blk.d1 * blk.d2 * blk.esz;
+    if (blk.d1 == blk.d2) { *eff = 1.0; return IPT_VARIANT_SQUARE_SWAP; }
+    const GcsGeom gg = gcs_geom(ctx, blk, lds_max_of(ctx), true, true);
+    const bool slice = bytes <= kBcLdsBlockBytes && variant_feasible(ctx, blk, IPT_VARIANT_SLICE_LDS);
+    if (slice && (bytes % 16) == 0) { *eff = 0.9; return IPT_VARIANT_SLICE_LDS; }
+    if (gg.ok && gg.vec) { *eff = 0.85; return IPT_VARIANT_GCS; }
+    if (gg.ok) { *eff = 0.5; return IPT_VARIANT_GCS; }
+    if (slice) { *eff = 0.5; return IPT_VARIANT_SLICE_LDS; }
+    return IPT_VARIANT_AUTO;
+}
+
+static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, BlockCyclePlan* best,
+                             double* best_cost) {
+    const int64_t esz = s.esz;
+    const int64_t L = cols ? s.d1 : s.d2, other = cols ? s.d2 : s.d1;
+    const int64_t tmax = std::min<int64_t>(L / 2, kBcMaxChunk / esz);
+    const int64_t tmin = (kBcMinChunk + esz - 1) / esz;
+    for (int64_t T = tmax; T >= tmin; T--) {
+        if (L % T || (T * esz) % 16) continue;
+        const int64_t nb = L / T;
+        if (nb * other >= kBcMaxChunks) break;  // smaller T only adds chunks
+        const Shape blk = cols ? make_shape(s.esz, T, other, s.d3 * nb)
+                               : make_shape(s.esz, other, T, s.d3 * nb);
+        double eff;
+        const int v = block_kernel(ctx, blk, &eff);
+        if (v == IPT_VARIANT_AUTO) continue;
+        const int64_t chunk = T * esz;
+        const int64_t ell = std::max<int64_t>(1, std::min<int64_t>(kBcMaxSeg, kBcStageBytes / chunk));
+        const double short_run = chunk < 2048 ? 1.0 + (2048.0 - chunk) / 4096.0 : 1.0;
+        const double cost = (1.0 + 1.5 / ell) * short_run + 1.0 / eff;
+        if (best->ok && cost >= *best_cost - 1e-9) continue;  // ties keep the larger chunk
+        best->ok = true;
+        best->chunk_first = cols;
+        best->T = T;
+        best->nb = nb;
+        best->other = other;
+        best->rc = cols ? other : nb;
+        best->cc = cols ? nb : other;
+        best->cv = chunk / 16;
+        best->ell = ell;
+        best->blk = blk;
+        best->block_variant = v;
+        *best_cost = cost;
+    }
+}
+
+BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s) {
+    BlockCyclePlan p{};
+    if (s.d1 == s.d2 || s.d1 < 2 || s.d2 < 2) return p;
+    double cost = 0;
+    plan_orientation(ctx, s, true, &p, &cost);
+    plan_orientation(ctx, s, false, &p, &cost);
+    return p;
+}
+
+// Move plans, built on the host once per (device, rc, cc, ell) and kept in
+// device memory until the process ends.
+struct MovePlan {
+    uint2* items = nullptr;    // (first move, moves)
+    uint32_t* msrc = nullptr;  // source position, or kHeadFlag | head slot
+    uint32_t* mdst = nullptr;  // destination position
+    uint32_t* hpos = nullptr;  // position of each saved head
+    uint32_t nitems = 0, nheads = 0;
+};
+static std::mutex g_plan_mu;
+static std::map<std::tuple<int, int64_t, int64_t, int64_t>, MovePlan> g_plans;
+
+template <typename V>
+static hipError_t upload(const std::vector<V>& h, V** d) {
+    *d = nullptr;
+    if (h.empty()) return hipSuccess;
+    hipError_t e = hipMalloc(d, h.size() * sizeof(V));
+    if (e == hipSuccess) e = hipMemcpy(*d, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice);
+    return e;
+}
+
+static ipt_status move_plan(const DeviceCtx* ctx, const BlockCyclePlan& p, hipStream_t st,
+                            MovePlan* out) {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    const auto key = std::make_tuple(ctx->dev, p.rc, p.cc, p.ell);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+        *out = it->second;
+        return IPT_OK;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return IPT_ERR_UNSUPPORTED;  // building a plan allocates
+    const uint32_t N = (uint32_t)(p.rc * p.cc), rc = (uint32_t)p.rc, cc = (uint32_t)p.cc;
+    const uint32_t ell = (uint32_t)p.ell;
+    std::vector<uint8_t> seen(N, 0);
+    std::vector<uint2> items;
+    std::vector<uint32_t> msrc, mdst, hpos, cyc;
+    uint32_t bundle = 0;  // moves in the open item of whole short cycles
+    auto close_bundle = [&] {
+        if (bundle) items.push_back(make_uint2((uint32_t)msrc.size() - bundle, bundle));
+        bundle = 0;
+    };
+    for (uint32_t q0 = 0; q0 < N; q0++) {
+        if (seen[q0]) continue;
+        cyc.clear();
+        uint32_t q = q0;
+        do {
+            seen[q] = 1;
+            cyc.push_back(q);
+            q = (q % cc) * rc + q / cc;
+        } while (q != q0);
+        const uint32_t n = (uint32_t)cyc.size();
+        if (n == 1) continue;  // fixed point
+        if (n <= ell) {
+            if (bundle + n > ell) close_bundle();
+            for (uint32_t j = 0; j < n; j++) {
+                msrc.push_back(cyc[j]);
+                mdst.push_back(cyc[(j + 1) % n]);
+            }
+            bundle += n;
+            continue;
+        }
+        close_bundle();
+        for (uint32_t a = 0; a < n; a += ell) {
+            const uint32_t len = std::min(ell, n - a);
+            items.push_back(make_uint2((uint32_t)msrc.size(), len));
+            for (uint32_t j = 0; j < len; j++) {
+                msrc.push_back(j == 0 ? (kHeadFlag | (uint32_t)hpos.size()) : cyc[a + j]);
+                mdst.push_back(cyc[(a + j + 1) % n]);
+            }
+            hpos.push_back(cyc[a]);
+        }
+    }
+    close_bundle();
+    MovePlan m;
+    m.nitems = (uint32_t)items.size();
+    m.nheads = (uint32_t)hpos.size();
+    hipError_t e = upload(items, &m.items);
+    if (e == hipSuccess) e = upload(msrc, &m.msrc);
+    if (e == hipSuccess) e = upload(mdst, &m.mdst);
+    if (e == hipSuccess) e = upload(hpos, &m.hpos);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(m.items);
+        (void)hipFree(m.msrc);
+        (void)hipFree(m.mdst);
+        (void)hipFree(m.hpos);
+        set_error("move plan upload", e);
+        return IPT_ERR_OUT_OF_MEMORY;
+    }
+    g_plans[key] = m;
+    *out = m;
+    return IPT_OK;
+}
+
+// The chunk step over slices [0, d3): saves the heads of cut cycles, then
+// runs the items, in groups of slices whose saved heads fit the scratch cap.
+static ipt_status chunk_step(DeviceCtx* ctx, void* data, const Shape& s, const BlockCyclePlan& p,
+                             hipStream_t st) {
+    MovePlan mp;
+    ipt_status stt = move_plan(ctx, p, st, &mp);
+    if (stt != IPT_OK) return stt;
+    if (mp.nitems == 0) return IPT_OK;
+    const uint64_t chunk_bytes = (uint64_t)p.cv * 16;
+    const uint64_t heads_per_slice = (uint64_t)mp.nheads * chunk_bytes;
+    if (heads_per_slice > kScratchCap) return IPT_ERR_UNSUPPORTED;
+    const uint64_t group =
+        heads_per_slice ? std::max<uint64_t>(1, kScratchCap / heads_per_slice) : (uint64_t)s.d3;
+    const uint64_t slice_vecs = (uint64_t)s.d1 * s.d2 * s.esz / 16;
+    const size_t lds = (size_t)p.ell * chunk_bytes;
+    {
+        // staging (up to kBcStageBytes) plus the static move lists exceed the
+        // default dynamic-LDS limit: raise it once per device
+        static std::set<int> raised;
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        if (!raised.count(ctx->dev)) {
+            IPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chunk_cycle),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)kBcStageBytes));
+            raised.insert(ctx->dev);
+        }
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    void* head = nullptr;
+    if (heads_per_slice) {
+        stt = arena_acquire(ctx->scratch,
+                            (size_t)(std::min<uint64_t>(group, (uint64_t)s.d3) * heads_per_slice), st,
+                            &head);
+        if (stt != IPT_OK) return stt;
+    }
+    const Div32 dcv = Div32::make((uint32_t)p.cv);
+    for (uint64_t s0 = 0; s0 < (uint64_t)s.d3; s0 += group) {
+        const uint64_t ns = std::min<uint64_t>(group, (uint64_t)s.d3 - s0);
+        uint4* d = static_cast<uint4*>(data) + s0 * slice_vecs;
+        if (mp.nheads) {
+            const uint64_t save_vecs = ns * mp.nheads * (uint64_t)p.cv;
+            const int gs = (int)std::min<uint64_t>((save_vecs + kChunkBlock * 4 - 1) / (kChunkBlock * 4),
+                                                   (uint64_t)ncu_of(ctx) * 8);
+            hipLaunchKernelGGL(k_chunk_save, dim3(gs), dim3(kChunkBlock), 0, st, (const uint4*)d,
+                               (uint4*)head, (const uint32_t*)mp.hpos, mp.nheads, ns, slice_vecs,
+                               (uint32_t)p.cv, dcv);
+            IPT_LAUNCH_CHECK("k_chunk_save");
+        }
+        const int gc = (int)std::min<uint64_t>(ns * mp.nitems, (uint64_t)ncu_of(ctx) * 8);
+        hipLaunchKernelGGL(k_chunk_cycle, dim3(gc), dim3(kChunkBlock), lds, st, d, (const uint4*)head,
+                           (const uint2*)mp.items, (const uint32_t*)mp.msrc,
+                           (const uint32_t*)mp.mdst, mp.nitems, mp.nheads, ns, slice_vecs,
+                           (uint32_t)p.cv, dcv);
+        IPT_LAUNCH_CHECK("k_chunk_cycle");
+    }
+    return heads_per_slice ? arena_release(ctx->scratch, st) : IPT_OK;
+}
+
+ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,
+                           hipStream_t st) {
+    const BlockCyclePlan p = block_cycle_plan(ctx, s);
+    if (!p.ok || (uintptr_t)data % 16 != 0) return IPT_ERR_UNSUPPORTED;
+    if (p.chunk_first) {
+        const ipt_status stt = chunk_step(ctx, data, s, p, st);
+        return stt == IPT_OK ? run_variant(ctx, data, p.blk, p.block_variant, t, st) : stt;
+    }
+    const ipt_status stt = run_variant(ctx, data, p.blk, p.block_variant, t, st);
+    return stt == IPT_OK ? chunk_step(ctx, data, s, p, st) : stt;
+}
+
+}  // namespace ipt

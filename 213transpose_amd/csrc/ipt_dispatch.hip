@@ -115,7 +115,7 @@ bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant) {
     case IPT_VARIANT_C2R:
     case IPT_VARIANT_R2C: return true;
     case IPT_VARIANT_GCS: return gcs_geom(ctx, s, lds_max_of(ctx)).ok;
-    case IPT_VARIANT_SKINNY: return skinny_plan(s).ok;
+    case IPT_VARIANT_BLOCK_CYCLE: return block_cycle_plan(ctx, s).ok;
     default: return false;
     }
 }
@@ -130,7 +130,7 @@ int choose_variant(const DeviceCtx* ctx, const Shape& s) {
     if (gcs_geom(ctx, s, lds_max_of(ctx)).ok) return IPT_VARIANT_GCS;
     if (slice_lds_fits(ctx, s)) return IPT_VARIANT_SLICE_LDS;
     if (s.d1 == s.d2) return IPT_VARIANT_SQUARE_SWAP;
-    if (skinny_plan(s).ok) return IPT_VARIANT_SKINNY;
+    if (block_cycle_plan(ctx, s).ok) return IPT_VARIANT_BLOCK_CYCLE;
     return s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
 }
 
@@ -441,7 +441,7 @@ static ipt_status run_typed(DeviceCtx* ctx, T* data, const Shape& s, int variant
     case IPT_VARIANT_C2R: return run_decomp<T>(ctx, data, s, true, t, st);
     case IPT_VARIANT_R2C: return run_decomp<T>(ctx, data, s, false, t, st);
     case IPT_VARIANT_GCS: return run_gcs<T>(ctx, data, s, t, st);
-    case IPT_VARIANT_SKINNY: return run_skinny(ctx, data, s, t, st);
+    case IPT_VARIANT_BLOCK_CYCLE: return run_block_cycle(ctx, data, s, t, st);
     default: return IPT_ERR_INVALID_ARG;
     }
 }

@@ -127,23 +127,30 @@ int64_t lds_max_of(const DeviceCtx* ctx);
 int ncu_of(const DeviceCtx* ctx);
 bool decomp_needs_scratch(const Shape& s, bool c2r, const Tune& t);
 
-// Skinny path (ipt_skinny.hip): one short side s, long side L = M * B.
-constexpr int64_t kSkinnyMaxShort = 128;      // largest short side
-constexpr int64_t kSkinnyBlockBytes = 64 * 1024;  // s x B block, one LDS tile
-constexpr int64_t kSkinnyMinChunk = 512;      // bytes per moved chunk
-constexpr int64_t kSkinnyStageBytes = 64 * 1024;  // chunks staged per segment
-constexpr uint32_t kSkinnyMaxSeg = 128;       // positions per segment
+// Block-cycle path (ipt_block_cycle.hip): chunks of T elements along one
+// side moved by cycle following, blocks of other x T transposed in one sweep.
+constexpr int64_t kBcMinChunk = 512;             // bytes per moved chunk, at least
+constexpr int64_t kBcMaxChunk = 32 * 1024;       // and at most (two fit one segment's LDS)
+constexpr int64_t kBcStageBytes = 64 * 1024;     // chunks staged per segment
+constexpr int64_t kBcLdsBlockBytes = 64 * 1024;  // blocks up to this go to the whole-slice kernel
+constexpr uint32_t kBcMaxSeg = 128;              // chunk moves per work item
+constexpr int64_t kBcMaxChunks = (int64_t)1 << 23;  // chunks per slice (host plan size)
 
-struct SkinnyPlan {
+struct BlockCyclePlan {
     bool ok = false;
-    bool tall = false;   // R > C (s = C); else s = R
-    int64_t s = 0, L = 0, B = 0, M = 0;
-    int64_t rc = 0, cc = 0;  // chunk matrix transposed by the chunk step
-    int64_t cv = 0;          // 16-byte vectors per chunk
-    int64_t ell = 0;         // chunks per segment (at most)
+    bool chunk_first = false;  // chunks along the columns: chunk step, then block step
+    int64_t T = 0;             // elements per chunk
+    int64_t nb = 0;            // blocks (chunks along the chunked side) per slice
+    int64_t other = 0;         // the side that is not chunked
+    int64_t rc = 0, cc = 0;    // chunk matrix transposed by the chunk step
+    int64_t cv = 0;            // 16-byte vectors per chunk
+    int64_t ell = 0;           // chunks per segment (at most)
+    Shape blk;                 // the block step as a 213 problem
+    int block_variant = 0;     // its one-sweep kernel
 };
-SkinnyPlan skinny_plan(const Shape& s);
-ipt_status run_skinny(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t, hipStream_t st);
+BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s);
+ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,
+                           hipStream_t st);
 
 template <typename Ix>
 ipt_status eval_map_host(int map_id, const HostPlan& h, const int64_t* i, const int64_t* j,

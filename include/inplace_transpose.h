@@ -92,12 +92,13 @@ typedef enum {
                                     stages a slice in LDS, then writes it
                                     transposed; one sweep for slices up to the
                                     GPU's resident LDS */
-    IPT_VARIANT_SKINNY = 6,      /* one short side s (<= 128): blocks of B rows
-                                    (or columns) of the long side transposed
-                                    in LDS, and the B-element chunks moved by
-                                    segmented cycle following (the skinny path
-                                    of src/skinny.cu:313-363, rebuilt); about
-                                    two sweeps */
+    IPT_VARIANT_BLOCK_CYCLE = 6, /* one side cut into chunks of T elements:
+                                    the chunks moved by segmented cycle
+                                    following, blocks of T x (other side)
+                                    transposed by a one-sweep kernel; about two
+                                    sweeps.  With a short side and LDS-sized
+                                    blocks it is the skinny path of
+                                    src/skinny.cu:313-363, rebuilt */
     IPT_VARIANT_COUNT = 7
 } ipt_variant;
 

@@ -53,12 +53,12 @@ def test_auto_dispatch_matches_oracle(ipt, oracle, esz):
         run_case(ipt, oracle, d1, d2, d3, esz, ipt.VARIANT_AUTO, 100 + k)
 
 
-@pytest.mark.parametrize("variant", ["c2r", "r2c", "slice_lds", "square_swap", "gcs", "skinny"])
+@pytest.mark.parametrize("variant", ["c2r", "r2c", "slice_lds", "square_swap", "gcs", "block_cycle"])
 @pytest.mark.parametrize("esz", [4, 8])
 def test_each_variant_matches_oracle(ipt, oracle, variant, esz):
     v = {"c2r": ipt.VARIANT_C2R, "r2c": ipt.VARIANT_R2C, "slice_lds": ipt.VARIANT_SLICE_LDS,
          "square_swap": ipt.VARIANT_SQUARE_SWAP, "gcs": ipt.VARIANT_GCS,
-         "skinny": ipt.VARIANT_SKINNY}[variant]
+         "block_cycle": ipt.VARIANT_BLOCK_CYCLE}[variant]
     ran = 0
     for k, (d1, d2, d3) in enumerate(SHAPES):
         if not ipt.feasible(v, esz, d1, d2, d3):
@@ -70,37 +70,40 @@ def test_each_variant_matches_oracle(ipt, oracle, variant, esz):
     assert ran > 0
 
 
-SKINNY_SHAPES = [  # (d1, d2, d3): wide (d2 short rows) and tall (d1 short columns)
+BLOCK_CYCLE_SHAPES = [  # (d1, d2, d3): skinny wide / tall, then large rectangles
     (65536, 32, 1), (32, 65536, 1), (7, 1 << 18, 2), (1 << 16, 5, 3), (128, 4096, 2),
     (4096, 128, 2), (3, 1 << 19, 1), (1 << 19, 3, 1), (2000, 64, 2), (64, 2000, 2),
     (1000, 3, 2), (3, 1000, 2), (6000, 100, 1), (100, 6000, 1),
+    (6144, 4096, 1), (4096, 6144, 1), (5000, 3000, 2), (3000, 5000, 1), (8192, 2048, 3),
+    (10007, 2000, 1), (2000, 10007, 1), (1500, 1000, 3),
 ]
 
 
 @pytest.mark.parametrize("esz", [4, 8])
-def test_skinny_path_matches_oracle(ipt, oracle, esz):
-    """Skinny path (block transposes + segmented cycle-following chunk
-    permutation): thousands of segments, several slices per call, chunks from
-    512 B to 8 KB; back-to-back calls reuse the cached segment tables."""
+def test_block_cycle_path_matches_oracle(ipt, oracle, esz):
+    """Block-cycle path (blocks transposed by a one-sweep kernel + segmented
+    cycle-following chunk permutation), skinny and large rectangular slices:
+    up to millions of chunks, several slices per call, every block kernel;
+    back-to-back calls reuse the cached segment tables."""
     ran = 0
-    for k, (d1, d2, d3) in enumerate(SKINNY_SHAPES):
-        if not ipt.feasible(ipt.VARIANT_SKINNY, esz, d1, d2, d3):
+    for k, (d1, d2, d3) in enumerate(BLOCK_CYCLE_SHAPES):
+        if not ipt.feasible(ipt.VARIANT_BLOCK_CYCLE, esz, d1, d2, d3):
             continue
         for rep in range(2):
-            run_case(ipt, oracle, d1, d2, d3, esz, ipt.VARIANT_SKINNY, 900 + 10 * k + rep)
+            run_case(ipt, oracle, d1, d2, d3, esz, ipt.VARIANT_BLOCK_CYCLE, 900 + 10 * k + rep)
         ran += 1
-    assert ran >= 10
+    assert ran >= 16
 
 
-def test_skinny_unaligned_data_falls_back(ipt, oracle):
-    """The skinny path needs 16-byte aligned data: forced, it reports
+def test_block_cycle_unaligned_data_falls_back(ipt, oracle):
+    """The block-cycle path needs 16-byte aligned data: forced, it reports
     unsupported; AUTO takes the decomposition instead."""
-    d1, d2, d3, esz = 7, 1 << 22, 1, 4  # beyond GCS: AUTO resolves to the skinny path
+    d1, d2, d3, esz = 7, 1 << 22, 1, 4  # beyond GCS: AUTO resolves to the block-cycle path
     a = oracle.random_bits(d1 * d2 * d3 + 1, esz, 990)
     t = to_dev(a)
     view = t[1:]  # 4-byte offset
     with pytest.raises(ipt.TransposeError):
-        ipt.transpose_variant(view, d1, d2, d3, ipt.VARIANT_SKINNY)
+        ipt.transpose_variant(view, d1, d2, d3, ipt.VARIANT_BLOCK_CYCLE)
     ipt.transpose(view, d1, d2, d3)
     torch.cuda.synchronize()
     got = to_host(t, esz)
