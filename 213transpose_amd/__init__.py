@@ -329,7 +329,7 @@ def uneven_shapes(count, lo=2, hi=4096, seed=0):
 
 # ---------------------------------------------------------------- selector
 
-N_CANDIDATES = 8
+N_CANDIDATES = 9
 MLP_IN, MLP_HIDDEN = 8, 64
 MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + \
     N_CANDIDATES * MLP_HIDDEN + N_CANDIDATES

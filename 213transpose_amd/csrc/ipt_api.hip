@@ -159,11 +159,10 @@ ipt_status validate(const void* data, int esz, int64_t d1, int64_t d2, int64_t d
 }
 
 // IPT_VARIANT_AUTO: the learned selector's candidate when a selector is
-// loaded and no tune is forced, else the heuristic's variant.  Where the
-// block-cycle path applies it replaces a decomposition choice (it costs
-// about two sweeps where C2R / R2C cost three or more); when it cannot run
-// (unaligned data, more saved chunks than the scratch cap, or stream capture
-// before the shape's first use) the decomposition runs instead.
+// loaded and no tune is forced, else the heuristic's variant.  When the
+// block-cycle path is chosen but cannot run (unaligned data, more saved
+// chunk heads than the scratch cap, or stream capture before the shape's
+// first use) the decomposition runs instead.
 ipt_status run_auto(DeviceCtx* ctx, void* data, const Shape& s, const ipt_tune* tune,
                     hipStream_t st) {
     int v;
@@ -176,13 +175,11 @@ ipt_status run_auto(DeviceCtx* ctx, void* data, const Shape& s, const ipt_tune* 
         v = choose_variant(ctx, s);
         t = resolve_tune(ctx, tune);
     }
-    if (v == IPT_VARIANT_C2R || v == IPT_VARIANT_R2C || v == IPT_VARIANT_BLOCK_CYCLE) {
-        if (block_cycle_plan(ctx, s).ok) {
-            const ipt_status r =
-                run_variant(ctx, data, s, IPT_VARIANT_BLOCK_CYCLE, resolve_tune(ctx, tune), st);
-            if (r != IPT_ERR_UNSUPPORTED) return r;
-        }
-        if (v == IPT_VARIANT_BLOCK_CYCLE) v = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
+    if (v == IPT_VARIANT_BLOCK_CYCLE) {
+        const ipt_status r = run_variant(ctx, data, s, v, t, st);
+        if (r != IPT_ERR_UNSUPPORTED) return r;
+        v = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
+        t = resolve_tune(ctx, tune);
     }
     return run_variant(ctx, data, s, v, t, st);
 }

@@ -104,8 +104,10 @@ int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t l
     for (int k = 0; k < 4; k++)
         if (route_ok[k] && (best < 0 || y[k] < y[best])) best = k;
     if (best >= 0) return best;
-    for (int k = 4; k < kNumCandidates; k++)
-        if (candidate_ok(ctx, s, k) && (best < 0 || y[k] < y[best])) best = k;
+    for (int k = 4; k < kNumCandidates; k++)  // separate launches; never the block-cycle path
+        if (kCandidates[k].variant != IPT_VARIANT_BLOCK_CYCLE && candidate_ok(ctx, s, k) &&
+            (best < 0 || y[k] < y[best]))
+            best = k;
     return best;
 }
 

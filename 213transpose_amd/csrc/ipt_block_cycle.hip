@@ -210,6 +210,23 @@ struct MovePlan {
 };
 static std::mutex g_plan_mu;
 static std::map<std::tuple<int, int64_t, int64_t, int64_t>, MovePlan> g_plans;
+static size_t g_plan_bytes = 0;  // device bytes held by g_plans
+
+// Past kBcPlanCacheBytes the cache is emptied (after a device synchronise:
+// queued kernels may still read the plans) before the next plan is added.
+static hipError_t drop_plans_locked() {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    for (auto& kv : g_plans) {
+        (void)hipFree(kv.second.items);
+        (void)hipFree(kv.second.msrc);
+        (void)hipFree(kv.second.mdst);
+        (void)hipFree(kv.second.hpos);
+    }
+    g_plans.clear();
+    g_plan_bytes = 0;
+    return hipSuccess;
+}
 
 template <typename V>
 static hipError_t upload(const std::vector<V>& h, V** d) {
@@ -274,6 +291,14 @@ static ipt_status move_plan(const DeviceCtx* ctx, const BlockCyclePlan& p, hipSt
         }
     }
     close_bundle();
+    const size_t bytes = items.size() * sizeof(uint2) + (msrc.size() + mdst.size() + hpos.size()) * 4;
+    if (g_plan_bytes + bytes > kBcPlanCacheBytes && !g_plans.empty()) {
+        const hipError_t e = drop_plans_locked();
+        if (e != hipSuccess) {
+            set_error("hipDeviceSynchronize", e);
+            return IPT_ERR_HIP;
+        }
+    }
     MovePlan m;
     m.nitems = (uint32_t)items.size();
     m.nheads = (uint32_t)hpos.size();
@@ -291,6 +316,7 @@ static ipt_status move_plan(const DeviceCtx* ctx, const BlockCyclePlan& p, hipSt
         return IPT_ERR_OUT_OF_MEMORY;
     }
     g_plans[key] = m;
+    g_plan_bytes += bytes;
     *out = m;
     return IPT_OK;
 }

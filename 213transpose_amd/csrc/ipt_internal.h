@@ -135,6 +135,7 @@ constexpr int64_t kBcStageBytes = 64 * 1024;     // chunks staged per segment
 constexpr int64_t kBcLdsBlockBytes = 64 * 1024;  // blocks up to this go to the whole-slice kernel
 constexpr uint32_t kBcMaxSeg = 128;              // chunk moves per work item
 constexpr int64_t kBcMaxChunks = (int64_t)1 << 23;  // chunks per slice (host plan size)
+constexpr size_t kBcPlanCacheBytes = (size_t)256 << 20;  // move plans kept per process
 
 struct BlockCyclePlan {
     bool ok = false;

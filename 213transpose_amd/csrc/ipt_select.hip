@@ -33,6 +33,7 @@ const Candidate kCandidates[kNumCandidates] = {
     {IPT_VARIANT_C2R, {0, 512, 16, 0}},
     {IPT_VARIANT_R2C, {0, 0, 0, 0}},
     {IPT_VARIANT_R2C, {0, 512, 16, 0}},
+    {IPT_VARIANT_BLOCK_CYCLE, {0, 0, 0, 0}},
 };
 
 int heuristic_candidate(int variant) {
@@ -41,6 +42,7 @@ int heuristic_candidate(int variant) {
     case IPT_VARIANT_SQUARE_SWAP: return 1;
     case IPT_VARIANT_GCS: return 2;
     case IPT_VARIANT_C2R: return 4;
+    case IPT_VARIANT_BLOCK_CYCLE: return 8;
     default: return 6;
     }
 }
@@ -321,6 +323,10 @@ ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
         if (!candidate_feasible(ctx, s, k)) continue;
         const Tune t = resolve_tune(ctx, &kCandidates[k].tune);
         st = run_variant(ctx, data, s, kCandidates[k].variant, t, stream);  // warm-up
+        if (st == IPT_ERR_UNSUPPORTED) {  // feasible on paper, declined at run time
+            st = IPT_OK;
+            continue;
+        }
         if (st != IPT_OK) break;
         e = hipEventRecord(a, stream);
         for (int r = 0; r < reps && st == IPT_OK && e == hipSuccess; r++)

@@ -166,7 +166,7 @@ ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream
  * candidate with the lowest prediction, for problems up to 1 GiB of payload
  * (the domain of its training sweep).  Without a loaded selector, and above
  * that size, the plain heuristic decides. */
-#define IPT_NUM_CANDIDATES 8
+#define IPT_NUM_CANDIDATES 9
 #define IPT_MLP_IN 8
 #define IPT_MLP_HIDDEN 64
 /* Number of fp32 parameters: W1, b1, W2, b2, W3, b3 (row-major, out x in). */

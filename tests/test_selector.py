@@ -14,9 +14,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DATA = os.path.join(ROOT, "213transpose_amd", "selector", "sweep.npz")
 WEIGHTS = os.path.join(ROOT, "213transpose_amd", "selector", "weights.f32")
 HELDOUT = 2000
+NOUT = 9  # candidates (IPT_NUM_CANDIDATES)
 
 
-def unpack(p, nin=8, nh=64, nout=8):
+def unpack(p, nin=8, nh=64, nout=NOUT):
     i = 0
     out = []
     for a, b in ((nh, nin), (nh, 1), (nh, nh), (nh, 1), (nout, nh), (nout, 1)):
@@ -53,9 +54,9 @@ def loss_grad64(p, x, y, mask):
 def problem(n, seed):
     rng = np.random.default_rng(seed)
     x = rng.uniform(0, 1, size=(n, 8)).astype(np.float32)
-    y = rng.normal(4, 2, size=(n, 8)).astype(np.float32)
-    mask = (rng.random((n, 8)) < 0.7).astype(np.float32)
-    p = (rng.normal(0, 0.3, size=8 * 64 + 64 + 64 * 64 + 64 + 8 * 64 + 8)).astype(np.float32)
+    y = rng.normal(4, 2, size=(n, NOUT)).astype(np.float32)
+    mask = (rng.random((n, NOUT)) < 0.7).astype(np.float32)
+    p = (rng.normal(0, 0.3, size=8 * 64 + 64 + 64 * 64 + 64 + NOUT * 64 + NOUT)).astype(np.float32)
     return x, y, mask, p
 
 
@@ -88,13 +89,16 @@ def test_shape_features(ipt):
 
 
 def test_candidates_and_heuristic_mapping(ipt):
+    assert ipt.N_CANDIDATES == NOUT
+    assert ipt.MLP_PARAMS == 8 * 64 + 64 + 64 * 64 + 64 + NOUT * 64 + NOUT
     seen = set()
     for k in range(ipt.N_CANDIDATES):
         v, _ = ipt.candidate(k)
         seen.add(v)
     assert seen == {ipt.VARIANT_SLICE_LDS, ipt.VARIANT_SQUARE_SWAP, ipt.VARIANT_GCS,
-                    ipt.VARIANT_C2R, ipt.VARIANT_R2C}
-    for shape in [(3000, 1000, 1), (512, 384, 64), (16384, 16384, 1), (7, 9, 1)]:
+                    ipt.VARIANT_C2R, ipt.VARIANT_R2C, ipt.VARIANT_BLOCK_CYCLE}
+    for shape in [(3000, 1000, 1), (512, 384, 64), (16384, 16384, 1), (7, 9, 1),
+                  (7, 1 << 22, 1), (16384, 8192, 1)]:
         k = ipt.heuristic_candidate(4, *shape)
         assert ipt.candidate(k)[0] == ipt.plan_query(4, *shape)["heuristic_variant"]
 
@@ -145,12 +149,12 @@ def test_gpu_inference_bit_identical_to_host(ipt):
 def test_gpu_training_fits_smooth_targets(ipt):
     rng = np.random.default_rng(6)
     x = rng.uniform(0, 1, size=(2000, 8)).astype(np.float32)
-    w = rng.normal(size=(8, 8))
+    w = rng.normal(size=(8, NOUT))
     y = (np.sin(x @ w) * 3 + 5).astype(np.float32)
     mask = np.ones_like(y)
     p0 = np.concatenate([rng.normal(0, np.sqrt(2 / a), size=a * b) if k % 2 == 0 else np.zeros(b)
                          for k, (a, b) in enumerate([(8, 64), (8, 64), (64, 64), (64, 64),
-                                                     (64, 8), (64, 8)])]).astype(np.float32)
+                                                     (64, NOUT), (64, NOUT)])]).astype(np.float32)
     loss0, _ = ipt.selector_grad(x, y, mask, p0)
     p1, loss1 = ipt.selector_train(x, y, mask, p0, epochs=600, lr=3e-3)
     assert loss1 < 0.05 * loss0, (loss0, loss1)
