@@ -230,9 +230,12 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
     const uint32_t nt = (N + TS - 1) / TS;
     const uint64_t pairs = (uint64_t)nt * (nt + 1) / 2;
     const bool vec = ((uint64_t)N * sizeof(T)) % 16 == 0 && (uintptr_t)data % 16 == 0;
-    auto k = vec ? k_square_swap<T, TS, true> : k_square_swap<T, TS, false>;
+    // Slices below 2^31 bytes go through a buffer resource (32-bit offsets),
+    // one grid round of resident workgroups (measured best on 16384^2 f32).
+    const bool rs = vec && (uint64_t)N * N * sizeof(T) < ((uint64_t)1 << 31);
+    auto k = rs ? k_square_swap_rs<T, TS> : vec ? k_square_swap<T, TS, true> : k_square_swap<T, TS, false>;
     int grid;
-    grid_for(ctx, k, kSwapBlock, 0, pairs * (uint64_t)s.d3, t.grid_mult, &grid);
+    grid_for(ctx, k, kSwapBlock, 0, pairs * (uint64_t)s.d3, rs ? 1 : t.grid_mult, &grid);
     hipLaunchKernelGGL(k, dim3(grid), dim3(kSwapBlock), 0, st, data, N, pairs, (uint64_t)s.d3);
     IPT_LAUNCH_CHECK("k_square_swap");
     return IPT_OK;

