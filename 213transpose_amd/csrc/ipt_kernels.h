@@ -21,10 +21,13 @@ IPT_HD uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
     type* name = reinterpret_cast<type*>(ipt_lds_raw_)
 
 // Copy n contiguous elements global -> LDS.  `l` must have the same address
-// phase modulo 16 as `g`; the aligned body then moves 16 bytes per lane.
+// phase modulo 16 as `g`; the aligned body then moves 16 bytes per lane,
+// U vectors per thread in flight (lanes past the end repeat the last vector:
+// same address, same value, no branches around the loads).
 template <typename T>
 __device__ __forceinline__ void copy_in(const T* __restrict__ g, T* l, uint32_t n) {
     constexpr uint32_t PER = 16 / sizeof(T);
+    constexpr uint32_t U = 8;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
     uint32_t head = (uint32_t)(((16u - ((uintptr_t)g & 15u)) & 15u) / sizeof(T));
     if (head > n) head = n;
@@ -32,7 +35,13 @@ __device__ __forceinline__ void copy_in(const T* __restrict__ g, T* l, uint32_t 
     const uint32_t nv = (n - head) / PER;
     const uint4* gv = reinterpret_cast<const uint4*>(g + head);
     uint4* lv = reinterpret_cast<uint4*>(l + head);
-    for (uint32_t v = tid; v < nv; v += nthr) lv[v] = gv[v];
+    for (uint32_t v0 = 0; v0 < nv; v0 += U * nthr) {
+        uint4 r[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) r[u] = gv[min(v0 + u * nthr + tid, nv - 1)];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) lv[min(v0 + u * nthr + tid, nv - 1)] = r[u];
+    }
     for (uint32_t e = head + nv * PER + tid; e < n; e += nthr) l[e] = g[e];
 }
 
@@ -276,8 +285,19 @@ k_col_gather_lds(T* __restrict__ data, Plan<Ix> p, uint64_t d3, uint32_t W, uint
         const uint64_t j0 = (w - sl * nstrips) * W;
         const uint32_t wn = (uint32_t)umin64(W, C - j0);
         T* base = data + sl * (uint64_t)R * C + j0;
-        if (jj < wn)
-            for (uint32_t i = threadIdx.x / W; i < R; i += rstep) lds[i * W + jj] = base[i * C + jj];
+        {
+            // U rows per thread in flight; lanes past the strip's last column
+            // or row repeat the last element (same address, same value).
+            constexpr uint32_t U = 8;
+            const uint32_t jc = min(jj, wn - 1);
+            for (uint32_t i0 = threadIdx.x / W; i0 < R; i0 += U * rstep) {
+                T r[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) r[u] = base[(uint64_t)min(i0 + u * rstep, R - 1) * C + jc];
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) lds[min(i0 + u * rstep, R - 1) * W + jc] = r[u];
+            }
+        }
         __syncthreads();
         if (jj < wn) {
             const Ix j = (Ix)(j0 + jj);
@@ -298,18 +318,30 @@ template <typename T, typename Ix, typename Map>
 __global__ void __launch_bounds__(256)
 k_col_gather_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<Ix> p,
                         uint32_t W, uint32_t nstrips, uint64_t first, uint64_t n) {
+    // U elements per thread in flight; lanes past the end repeat the last
+    // element, and columns past C (a partial last strip) read column C - 1
+    // into scratch slots the copy back never reads.
+    constexpr uint32_t U = 4;
     const uint64_t R = p.R, C = p.C;
     const uint64_t per_item = R * W;
     const uint64_t total = n * per_item;
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t wl = e / per_item, r = e - wl * per_item;
-        const uint64_t i = r / W, jj = r - i * W;
-        const uint64_t w = first + wl, sl = w / nstrips;
-        const uint64_t j = (w - sl * nstrips) * W + jj;
-        if (j >= C) continue;
-        const T* base = data + sl * R * C;
-        scratch[e] = base[(uint64_t)Map::src(p, (Ix)i, (Ix)j) * C + j];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < total; e0 += stride * U) {
+        T r[U];
+        uint64_t at[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint64_t e = umin64(e0 + u * stride, total - 1);
+            const uint64_t wl = e / per_item, rem = e - wl * per_item;
+            const uint64_t i = rem / W, jj = rem - i * W;
+            const uint64_t w = first + wl, sl = w / nstrips;
+            const uint64_t j = umin64((w - sl * nstrips) * W + jj, C - 1);
+            const T* base = data + sl * R * C;
+            at[u] = e;
+            r[u] = base[(uint64_t)Map::src(p, (Ix)i, (Ix)j) * C + j];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) scratch[at[u]] = r[u];
     }
 }
 
@@ -317,16 +349,28 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_col_scratch_back(T* __restrict__ data, const T* __restrict__ scratch, uint64_t R, uint64_t C,
                    uint32_t W, uint32_t nstrips, uint64_t first, uint64_t n) {
+    constexpr uint32_t U = 4;  // elements per thread in flight
     const uint64_t per_item = R * W;
     const uint64_t total = n * per_item;
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t wl = e / per_item, r = e - wl * per_item;
-        const uint64_t i = r / W, jj = r - i * W;
-        const uint64_t w = first + wl, sl = w / nstrips;
-        const uint64_t j = (w - sl * nstrips) * W + jj;
-        if (j >= C) continue;
-        data[sl * R * C + i * C + j] = scratch[e];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < total; e0 += stride * U) {
+        T r[U];
+        uint64_t dst[U];
+        bool ok[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint64_t e = umin64(e0 + u * stride, total - 1);
+            const uint64_t wl = e / per_item, rem = e - wl * per_item;
+            const uint64_t i = rem / W, jj = rem - i * W;
+            const uint64_t w = first + wl, sl = w / nstrips;
+            const uint64_t j = (w - sl * nstrips) * W + jj;
+            ok[u] = j < C;
+            dst[u] = sl * R * C + i * C + j;
+            r[u] = scratch[e];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++)
+            if (ok[u]) data[dst[u]] = r[u];
     }
 }
 
@@ -335,14 +379,24 @@ template <typename T, typename Ix, typename Map>
 __global__ void __launch_bounds__(256)
 k_row_gather_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<Ix> p,
                         uint64_t g0, uint64_t n) {
+    constexpr uint32_t U = 4;  // elements per thread in flight; past the end: the last again
     const uint64_t C = p.C;
     const uint64_t total = n * C;
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t r = e / C, j = e - r * C;
-        const uint64_t g = g0 + r;
-        const Ix i = (Ix)(g % (uint64_t)p.R);
-        scratch[e] = data[g * C + (uint64_t)Map::src(p, i, (Ix)j)];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < total; e0 += stride * U) {
+        T r[U];
+        uint64_t at[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint64_t e = umin64(e0 + u * stride, total - 1);
+            const uint64_t rr = e / C, j = e - rr * C;
+            const uint64_t g = g0 + rr;
+            const Ix i = (Ix)(g % (uint64_t)p.R);
+            at[u] = e;
+            r[u] = data[g * C + (uint64_t)Map::src(p, i, (Ix)j)];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) scratch[at[u]] = r[u];
     }
 }
 
