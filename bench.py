@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--cands", default=None,
                     help="--profile: selector candidate per workload tensor, comma-separated")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="build the uneven batch and run its graph a few times (for a profiler)")
     ap.add_argument("--profile", action="store_true",
                     help="only run workload steps and the calibration copies (for profilers)")
     return ap.parse_args()
@@ -225,9 +227,8 @@ def measure_traffic(match, shape_esz, cands):
     return totals, None
 
 
-def batch_leg(torch, ipt, count=4096, seed=4, reps=3):
-    """Plain per-call heuristic dispatch vs the learned selector + one planned,
-    graph-captured batch, on the uneven-batch workload."""
+def uneven_items(torch, ipt, count=4096, seed=4):
+    """The uneven-batch workload: (tensor, d1, d2, d3) views into one buffer."""
     shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(count, 2, 4096, seed=seed)]
     offs, total = [], 0
     for d1, d2, d3, esz in shapes:
@@ -241,6 +242,25 @@ def batch_leg(torch, ipt, count=4096, seed=4, reps=3):
         items.append((raw[off:off + n * esz].view(torch.int32 if esz == 4 else torch.int64),
                       d1, d2, d3))
     payload = sum(d1 * d2 * d3 * esz for d1, d2, d3, esz in shapes)
+    return buf, items, payload
+
+
+def profile_batch_main(args, torch, ipt):
+    """The uneven batch's graph, run warmup + steps times (runs under rocprofv3)."""
+    buf, items, _ = uneven_items(torch, ipt)
+    b = ipt.Batch(items)
+    for _ in range(args.warmup + args.steps):
+        b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    b.close()
+    print("profile run done", flush=True)
+
+
+def batch_leg(torch, ipt, count=4096, seed=4, reps=3):
+    """Plain per-call heuristic dispatch vs the learned selector + one planned,
+    graph-captured batch, on the uneven-batch workload."""
+    buf, items, payload = uneven_items(torch, ipt, count, seed)
 
     def per_call():
         for t, d1, d2, d3 in items:
@@ -264,7 +284,7 @@ def batch_leg(torch, ipt, count=4096, seed=4, reps=3):
     learned = ipt.load_default_selector()
     ms_loop_l = time_calls(torch, lambda _: per_call(), None, reps)
     ms_batch_l, plan_ms, info = batched()
-    del items, raw, buf
+    del items, buf
     torch.cuda.empty_cache()
     return {"matrices": count, "payload_bytes": payload, "selector_loaded": learned,
             "ms_per_call_heuristic": round(ms_loop_h, 3),
@@ -305,6 +325,9 @@ def main():
     torch.cuda.set_device(0)
     if args.profile:
         profile_main(args, torch, ipt)
+        return
+    if args.profile_batch:
+        profile_batch_main(args, torch, ipt)
         return
     tensors = make_tensors(torch)
 
