@@ -16,11 +16,23 @@
 
 namespace ipt {
 
+// Work queue: a workgroup takes the next item (ctl[kQueueNext]) whenever it
+// is free, so items of uneven cost balance across the grid.  ctl[kQueueDone]
+// counts workgroups that found the queue empty; the last one zeroes both
+// words (zero at rest, so graph replays start from an empty count).
+constexpr int kQueueNext = 8, kQueueDone = 9;
+
 __global__ void __launch_bounds__(kBatchBlock)
-k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items, uint64_t nitems,
+k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items, uint32_t nitems,
         uint32_t* __restrict__ ctl, uint32_t* __restrict__ bars) {
     IPT_DYN_LDS(unsigned char, lds);
-    for (uint64_t s = blockIdx.x; s < nitems; s += gridDim.x) {
+    __shared__ uint32_t next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomic_add_agent(&ctl[kQueueNext], 1u);
+        __syncthreads();
+        const uint32_t s = next;
+        __syncthreads();  // `next` is rewritten by the next take
+        if (s >= nitems) break;
         const BatchItem it = items[s];
         const BatchInst I = insts[it.inst];
         if (I.esz == 4)
@@ -28,6 +40,10 @@ k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items
         else
             batch_item<uint64_t>(I, it, ctl, bars, reinterpret_cast<uint64_t*>(lds));
         __syncthreads();  // LDS is reused by the next item
+    }
+    if (threadIdx.x == 0 && atomic_add_agent(&ctl[kQueueDone], 1u) == gridDim.x - 1) {
+        atomic_store_agent(&ctl[kQueueNext], 0u);
+        atomic_store_agent(&ctl[kQueueDone], 0u);
     }
 }
 
@@ -114,7 +130,7 @@ int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t l
 ipt_status enqueue(ipt_batch* b, hipStream_t st) {
     if (b->nitems) {
         hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBatchBlock), b->lds, st, b->d_insts,
-                           b->d_items, b->nitems, b->d_sync, b->d_sync + 32);
+                           b->d_items, (uint32_t)b->nitems, b->d_sync, b->d_sync + 32);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             set_error("k_batch", e);
@@ -288,6 +304,10 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     items.insert(items.end(), swap_items.begin(), swap_items.end());
     for (const auto& x : slice_sorted) items.push_back(x.second);
 
+    if (items.size() >= ((uint64_t)1 << 32)) {  // the work queue counts in 32 bits
+        release(b);
+        return IPT_ERR_TOO_LARGE;
+    }
     b->nitems = items.size();
     b->lds = std::max<size_t>(lds_need, 16);
     b->grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)grid, b->nitems));

@@ -913,9 +913,13 @@ namespace ipt {
 //             on the vector path, gcs_body on the scalar path);
 //             the g members of a slice occupy consecutive items
 //   BK_SWAP   tile pair `sub` of one square slice (swap_body)
-// Workgroup b takes items b, b + grid, ... (static round-robin), so the
-// deadlock argument of k_gcs holds unchanged (every g <= grid, other items
-// never wait).
+// Workgroups take items from a queue in increasing order, one at a time, and
+// a workgroup never holds an item it is not running.  Deadlock freedom: the
+// grid never exceeds the resident workgroups and every g <= grid; each group
+// before the queue's frontier has all its members taken, hence running, so
+// the earliest unfinished group completes; the frontier group gets its
+// remaining members from workgroups freed as earlier groups complete (all
+// grid workgroups would suffice).  Only group members ever wait.
 enum BatchKind : uint32_t { BK_SLICE = 0, BK_GCS_VEC = 1, BK_GCS_SCALAR = 2, BK_SWAP = 3,
                             BK_SWAP_VEC = 4 };
 
