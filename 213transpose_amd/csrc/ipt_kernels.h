@@ -769,54 +769,68 @@ struct GcsTile {
 // Registers each thread holds across the barrier on the scalar path.
 constexpr int gcs_scalar_rv(int esz) { return esz == 4 ? 64 : 32; }
 
-// Element v of tile t (input layout): its offset, false when outside.
-template <int TS>
-__device__ __forceinline__ bool gcs_src(uint32_t t, uint32_t v, uint32_t R, uint32_t C, Div32 dntj,
-                                        uint64_t* off) {
-    const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
-    const uint32_t r = ti * TS + v / TS, c = tj * TS + v % TS;
-    *off = (uint64_t)r * C + c;
-    return r < R && c < C;
+// Element loads and stores through the slice's buffer resource: 4 or 8
+// bytes, an offset of kOutside loads nothing (zero) and stores nothing.
+template <typename T>
+__device__ __forceinline__ T buf_load(Rsrc rs, uint32_t off) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+    } else {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+        return ((T)v[1] << 32) | v[0];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void buf_store(Rsrc rs, uint32_t off, T x) {
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(x, rs, off, 0, kStoreDevice);
+    } else {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 v = {(uint32_t)x, (uint32_t)(x >> 32)};
+        __builtin_amdgcn_raw_buffer_store_b64(v, rs, off, 0, kStoreDevice);
+    }
 }
 
-// LDS index of element v of staged tile slot k (input layout, padded).
-template <int TS>
-__device__ __forceinline__ uint32_t gcs_slot(uint32_t k, uint32_t v) {
-    return k * GcsTile<uint32_t, TS>::TILE + (v / TS) * (TS + 1) + v % TS;
-}
+// Thread geometry of the scalar path.  A tile has VPT = TS * TS elements;
+// with BLOCK >= VPT a workgroup covers S = BLOCK / VPT tiles per step, one
+// element each, else E = VPT / BLOCK elements of one tile.  The element a
+// thread moves (its row y and column x in the tile) is the same for every
+// tile, so only each tile's origin is computed per tile.
+template <int TS, int BLOCK>
+struct GcsLanes {
+    static constexpr uint32_t VPT = TS * TS;
+    static constexpr uint32_t E = VPT >= (uint32_t)BLOCK ? VPT / BLOCK : 1;
+    static constexpr uint32_t S = VPT >= (uint32_t)BLOCK ? 1 : BLOCK / VPT;
+    uint32_t ts;     // tile within a step
+    uint32_t v0;     // first element (then v0 + e * BLOCK, e < E)
+    __device__ __forceinline__ GcsLanes() {
+        ts = S > 1 ? threadIdx.x / VPT : 0;
+        v0 = S > 1 ? threadIdx.x % VPT : threadIdx.x;
+    }
+};
 
-// Write staged tiles [k0, k0 + n) from LDS slots [0, n) transposed: output
-// row = input column.  Four elements per thread are read from LDS before
-// the first store is issued.
+// Writes staged tiles [k0, k0 + n) from LDS slots [0, n) transposed
+// (output row = input column), each thread the same element of every tile.
 template <typename T, int TS, int BLOCK>
-__device__ __forceinline__ void gcs_store(T* __restrict__ base, const T* lds, uint32_t R, uint32_t C,
-                                          Div32 dntj, uint32_t m, uint32_t g, uint32_t k0,
-                                          uint32_t n) {
+__device__ __forceinline__ void gcs_store(Rsrc rs, const T* lds, uint32_t R, uint32_t C, Div32 dntj,
+                                          uint32_t m, uint32_t g, uint32_t k0, uint32_t n) {
     using G = GcsTile<T, TS>;
-    constexpr uint32_t W = 4;
-    const uint32_t total = n * G::VPT;
-    for (uint32_t i0 = 0; i0 < total; i0 += W * BLOCK) {
-        T u[W];
-        uint64_t off[W];
-        bool ok[W];
+    using L = GcsLanes<TS, BLOCK>;
+    const L ln;
+    for (uint32_t s0 = 0; s0 < n; s0 += L::S) {
+        const uint32_t k = s0 + ln.ts;
+        const uint32_t t = m + (k0 + k) * g;
+        const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
+        const bool live = k < n;
 #pragma unroll
-        for (uint32_t w = 0; w < W; w++) {
-            const uint32_t idx = i0 + w * BLOCK + threadIdx.x;
-            ok[w] = false;
-            if (idx < total) {
-                const uint32_t k = idx / G::VPT, x = idx % G::VPT;
-                const uint32_t t = m + (k0 + k) * g;
-                const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
-                const uint32_t cc = x / TS, rr = x % TS;  // output row cc, column rr
-                const uint32_t orow = tj * TS + cc, ocol = ti * TS + rr;
-                ok[w] = orow < C && ocol < R;
-                off[w] = (uint64_t)orow * R + ocol;
-                u[w] = lds[k * G::TILE + rr * G::TP + cc];
-            }
+        for (uint32_t e = 0; e < L::E; e++) {
+            const uint32_t v = ln.v0 + e * BLOCK;
+            const uint32_t cc = v / TS, rr = v % TS;  // output row cc, column rr of the tile
+            const uint32_t orow = tj * TS + cc, ocol = ti * TS + rr;
+            const bool ok = live && orow < C && ocol < R;
+            const T x = lds[(live ? k : 0) * G::TILE + rr * G::TP + cc];
+            buf_store<T>(rs, ok ? (orow * R + ocol) * (uint32_t)sizeof(T) : kOutside, x);
         }
-#pragma unroll
-        for (uint32_t w = 0; w < W; w++)
-            if (ok[w]) base[off[w]] = u[w];
     }
 }
 
@@ -824,62 +838,73 @@ __device__ __forceinline__ void gcs_store(T* __restrict__ base, const T* lds, ui
 // slice at `base`.  Member m owns tiles t = m + k*g, k < nk; the first kl of
 // them are staged in LDS, the remaining nk - kl (at most RV * BLOCK / VPT)
 // stay in registers across the group barrier and go through LDS after the
-// LDS-staged tiles have been written.  Loads and LDS writes are free of
-// branches (a lane with nothing to do loads element 0 and writes the pad
-// element of slot 0, which is never read), so the compiler keeps every
-// load of a batch in flight.
+// LDS-staged tiles have been written.  Loads go through the slice's buffer
+// resource (lanes with nothing to load use kOutside), so every load of a
+// batch is in flight with no branches; a lane with nothing to stage writes
+// the pad element of slot 0, which is never read.
 template <typename T, int TS, int BLOCK, int RV>
 __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
                                          uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
                                          uint32_t m, uint32_t* bar, uint32_t* ctl, T* lds) {
     using G = GcsTile<T, TS>;
-    constexpr uint32_t U = 32;  // LDS-bound elements in flight per thread
-    constexpr uint32_t PAD = TS;  // slot 0, row 0, pad column
+    using L = GcsLanes<TS, BLOCK>;
+    constexpr uint32_t U = 32;          // LDS-bound elements in flight per thread
+    constexpr uint32_t PAD = TS;        // slot 0, row 0, pad column
+    constexpr uint32_t RT = RV / L::E * L::S;  // register tiles (at most)
+    const L ln;
     const uint32_t nk = m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0;
     const uint32_t kl = min(nk, KL), kr = nk - kl;
+    const Rsrc rs = make_rsrc(base, R * C * (uint32_t)sizeof(T));
+    // input offset (bytes) of this thread's element e of member tile k
+    auto src = [&](uint32_t k, uint32_t e, bool live) -> uint32_t {
+        const uint32_t t = m + k * g;
+        const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
+        const uint32_t v = ln.v0 + e * BLOCK;
+        const uint32_t r = ti * TS + v / TS, c = tj * TS + v % TS;
+        return live && r < R && c < C ? (r * C + c) * (uint32_t)sizeof(T) : kOutside;
+    };
+    auto slot = [&](uint32_t k, uint32_t e) -> uint32_t {
+        const uint32_t v = ln.v0 + e * BLOCK;
+        return k * G::TILE + (v / TS) * G::TP + v % TS;
+    };
 
+    // register-held tiles kl .. nk - 1: RV elements per thread, issued first
     T rreg[RV];
-    const uint32_t rtotal = kr * G::VPT;
     if (kr) {
 #pragma unroll
         for (uint32_t u = 0; u < RV; u++) {
-            const uint32_t idx = u * BLOCK + threadIdx.x;
-            uint64_t off;
-            const bool ok = idx < rtotal &&
-                            gcs_src<TS>(m + (kl + idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off);
-            rreg[u] = base[ok ? off : 0];
+            const uint32_t k = (u / L::E) * L::S + ln.ts, e = u % L::E;
+            rreg[u] = buf_load<T>(rs, src(kl + k, e, k < kr));
         }
     }
-    const uint32_t ltotal = kl * G::VPT;
-    for (uint32_t b0 = 0; b0 < ltotal; b0 += U * BLOCK) {
+    // LDS-staged tiles 0 .. kl - 1, U elements per thread per batch
+    for (uint32_t k0 = 0; k0 < kl; k0 += U / L::E * L::S) {
         T reg[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint32_t idx = b0 + u * BLOCK + threadIdx.x;
-            uint64_t off;
-            const bool ok = idx < ltotal &&
-                            gcs_src<TS>(m + (idx / G::VPT) * g, idx % G::VPT, R, C, dntj, &off);
-            reg[u] = base[ok ? off : 0];
+            const uint32_t k = k0 + (u / L::E) * L::S + ln.ts, e = u % L::E;
+            reg[u] = buf_load<T>(rs, src(k, e, k < kl));
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint32_t idx = b0 + u * BLOCK + threadIdx.x;
-            lds[idx < ltotal ? gcs_slot<TS>(idx / G::VPT, idx % G::VPT) : PAD] = reg[u];
+            const uint32_t k = k0 + (u / L::E) * L::S + ln.ts, e = u % L::E;
+            lds[k < kl ? slot(k, e) : PAD] = reg[u];
         }
     }
     group_sync(bar, ctl, m, g);
-    gcs_store<T, TS, BLOCK>(base, lds, R, C, dntj, m, g, 0, kl);
+    gcs_store<T, TS, BLOCK>(rs, lds, R, C, dntj, m, g, 0, kl);
     if (kr) {
         __syncthreads();
 #pragma unroll
         for (uint32_t u = 0; u < RV; u++) {
-            const uint32_t idx = u * BLOCK + threadIdx.x;
-            lds[idx < rtotal ? gcs_slot<TS>(idx / G::VPT, idx % G::VPT) : PAD] = rreg[u];
+            const uint32_t k = (u / L::E) * L::S + ln.ts, e = u % L::E;
+            lds[k < kr ? slot(k, e) : PAD] = rreg[u];
         }
         __syncthreads();
-        gcs_store<T, TS, BLOCK>(base, lds, R, C, dntj, m, g, kl, kr);
+        gcs_store<T, TS, BLOCK>(rs, lds, R, C, dntj, m, g, kl, kr);
     }
     if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
+    (void)RT;
 }
 
 // Scalar-path GCS kernel: slot s is member s % g of slice s / g.
