@@ -68,7 +68,8 @@ class MatrixDesc(ctypes.Structure):
 class BatchInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("count", "skipped", "n_instances", "n_big",
                                                "n_slice_items", "n_gcs_items", "n_swap_items",
-                                               "grid", "lds_bytes")] + [("bytes", ctypes.c_uint64)]
+                                               "grid", "lds_bytes")] + [
+        ("bytes", ctypes.c_uint64), ("n_block_cycle", ctypes.c_int64)]
 
 
 _i64, _vp, _int = ctypes.c_int64, ctypes.c_void_p, ctypes.c_int

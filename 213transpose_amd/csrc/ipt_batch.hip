@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <new>
 #include <vector>
 
@@ -67,6 +69,16 @@ struct ipt_batch_s {
         Tune tune;
     };
     std::vector<Big> big;  // instances run by the per-call launchers
+    // Chunk steps of block-cycle instances (their blocks are k_batch items).
+    struct Chunk {
+        void* data;
+        uint64_t nslices, slice_vecs, group;
+        BlockCyclePlan plan;
+        size_t moves;  // index into `moves`
+    };
+    std::vector<Chunk> pre, post;  // before / after k_batch
+    std::vector<ChunkMoves> moves;  // owned by the batch
+    void* d_head = nullptr;         // saved heads, shared by the chunk steps in turn
     Tune tune{};
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
@@ -91,7 +103,8 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
     case IPT_VARIANT_GCS: return gcs_ok ? Route::Gcs : Route::Big;
     case IPT_VARIANT_SQUARE_SWAP: return s.d1 == s.d2 ? Route::Swap : Route::Big;
     case IPT_VARIANT_C2R:
-    case IPT_VARIANT_R2C: *big_variant = forced; return Route::Big;
+    case IPT_VARIANT_R2C:
+    case IPT_VARIANT_BLOCK_CYCLE: *big_variant = forced; return Route::Big;
     default: break;
     }
     if (slice_bytes <= lds_cap) return Route::Slice;
@@ -105,9 +118,10 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
 // Candidate for an AUTO instance of a batch: a measured choice if the
 // autotuner recorded one; else, with predictions y, the fastest predicted
 // route of the persistent kernel (whole slice, cooperative group, square
-// swap) when any fits, and only otherwise the fastest predicted C2R / R2C
-// candidate (separate launches cost far more inside a batch than their
-// standalone timings show).  -1 = let plan_instance's heuristic decide.
+// swap) when any fits, and only otherwise the fastest predicted candidate
+// with separate launches (block cycle, C2R, R2C: such launches cost more
+// inside a batch than their standalone timings show).  -1 = let
+// plan_instance's heuristic decide.
 int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t lds_cap, int grid) {
     const int tuned = tuned_candidate(s);
     if (tuned >= 0) return tuned;
@@ -120,14 +134,23 @@ int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t l
     for (int k = 0; k < 4; k++)
         if (route_ok[k] && (best < 0 || y[k] < y[best])) best = k;
     if (best >= 0) return best;
-    for (int k = 4; k < kNumCandidates; k++)  // separate launches; never the block-cycle path
-        if (kCandidates[k].variant != IPT_VARIANT_BLOCK_CYCLE && candidate_ok(ctx, s, k) &&
-            (best < 0 || y[k] < y[best]))
-            best = k;
+    for (int k = 4; k < kNumCandidates; k++)
+        if (candidate_ok(ctx, s, k) && (best < 0 || y[k] < y[best])) best = k;
     return best;
 }
 
+ipt_status run_chunks(ipt_batch* b, const std::vector<ipt_batch::Chunk>& cs, hipStream_t st) {
+    for (const auto& c : cs) {
+        const ipt_status r = chunk_step_launch(b->ctx, c.data, c.nslices, c.slice_vecs, c.plan,
+                                               b->moves[c.moves], b->d_head, c.group, st);
+        if (r != IPT_OK) return r;
+    }
+    return IPT_OK;
+}
+
 ipt_status enqueue(ipt_batch* b, hipStream_t st) {
+    ipt_status r = run_chunks(b, b->pre, st);
+    if (r != IPT_OK) return r;
     if (b->nitems) {
         hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBatchBlock), b->lds, st, b->d_insts,
                            b->d_items, (uint32_t)b->nitems, b->d_sync, b->d_sync + 32);
@@ -137,8 +160,10 @@ ipt_status enqueue(ipt_batch* b, hipStream_t st) {
             return IPT_ERR_HIP;
         }
     }
+    r = run_chunks(b, b->post, st);
+    if (r != IPT_OK) return r;
     for (const auto& x : b->big) {
-        ipt_status r = run_variant(b->ctx, x.data, x.s, x.variant, x.tune, st);
+        r = run_variant(b->ctx, x.data, x.s, x.variant, x.tune, st);
         if (r != IPT_OK) return r;
     }
     return IPT_OK;
@@ -151,6 +176,8 @@ void release(ipt_batch* b) {
     if (b->d_insts) (void)hipFree(b->d_insts);
     if (b->d_items) (void)hipFree(b->d_items);
     if (b->d_sync) (void)hipFree(b->d_sync);
+    for (auto& m : b->moves) chunk_moves_free(&m);
+    if (b->d_head) (void)hipFree(b->d_head);
     delete b;
 }
 
@@ -201,6 +228,94 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     std::vector<std::pair<uint64_t, BatchItem>> slice_sorted;
     uint64_t bar_words_total = 0;
     size_t lds_need = 0;
+    // Items of one instance on a persistent-kernel route.
+    auto add_items = [&](void* data, const Shape& s, Route p) -> ipt_status {
+        BatchInst I{};
+        I.data = (uint64_t)(uintptr_t)data;
+        I.R = (uint32_t)s.d2;
+        I.C = (uint32_t)s.d1;
+        I.esz = (uint32_t)s.esz;
+        const uint32_t idx = (uint32_t)insts.size();
+        if (p == Route::Slice) {
+            I.kind = BK_SLICE;
+            I.P = (uint32_t)slice_pitch(s.d1);
+            I.dS = Div32::make(I.R * I.C);
+            I.dC = Div32::make(I.C);
+            I.dR = Div32::make(I.R);
+            const uint64_t slice_bytes = (uint64_t)I.R * I.P * s.esz;
+            const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(
+                (uint64_t)b->tune.lds_bytes / slice_bytes, (uint64_t)lds_cap / slice_bytes));
+            lds_need = std::max<size_t>(lds_need, std::min<uint64_t>(per, (uint64_t)s.d3) * slice_bytes);
+            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl += per) {
+                BatchItem it{idx, (uint32_t)std::min<uint64_t>(per, (uint64_t)s.d3 - sl), sl};
+                slice_sorted.push_back({slice_bytes * it.sub, it});
+            }
+            b->info.n_slice_items += ((uint64_t)s.d3 + per - 1) / per;
+        } else if (p == Route::Gcs) {
+            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, ((uintptr_t)data % 16) == 0);
+            I.kind = gg.vec ? BK_GCS_VEC : BK_GCS_SCALAR;
+            I.g = gg.g;
+            I.K = gg.K;
+            I.KL = gg.KL;
+            I.ntiles = gg.ntiles;
+            I.dntj = Div32::make(gg.ntj);
+            I.bar = bar_words_total;
+            bar_words_total += bar_words(gg.g) * (uint64_t)s.d3;
+            lds_need = std::max(lds_need, gg.lds);
+            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl++)
+                for (uint32_t m = 0; m < gg.g; m++) gcs_items.push_back({idx, m, sl});
+            b->info.n_gcs_items += (uint64_t)gg.g * s.d3;
+        } else {
+            const uint64_t TS = s.esz == 4 ? 64 : 32;
+            I.kind = ((uint64_t)s.d1 * s.esz) % 16 == 0 && (uintptr_t)data % 16 == 0 ? BK_SWAP_VEC
+                                                                                      : BK_SWAP;
+            const uint64_t nt = ((uint64_t)s.d1 + TS - 1) / TS;
+            const uint64_t pairs = nt * (nt + 1) / 2;
+            if (pairs >= ((uint64_t)1 << 32)) return IPT_ERR_TOO_LARGE;
+            lds_need = std::max<size_t>(lds_need, 2 * TS * (TS + 1) * s.esz);
+            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl++)
+                for (uint64_t t = 0; t < pairs; t++) swap_items.push_back({idx, (uint32_t)t, sl});
+            b->info.n_swap_items += pairs * s.d3;
+        }
+        insts.push_back(I);
+        return IPT_OK;
+    };
+    // A block-cycle instance: its blocks on a persistent-kernel route, its
+    // chunk step recorded with move lists of its own (shared by instances of
+    // one chunk-matrix shape).  false = the instance cannot go this way.
+    std::map<std::tuple<int64_t, int64_t, int64_t>, size_t> moves_of;
+    size_t head_bytes = 0;
+    auto add_block_cycle = [&](void* data, const Shape& s, ipt_status* err) -> bool {
+        *err = IPT_OK;
+        if ((uintptr_t)data % 16 != 0) return false;
+        const BlockCyclePlan bp = block_cycle_plan(ctx, s);
+        if (!bp.ok) return false;
+        int bv = IPT_VARIANT_AUTO;
+        const Route rb = plan_instance(ctx, bp.blk, bp.block_variant, lds_cap, grid, b->tune, &bv);
+        if (rb == Route::Big || rb == Route::Skip) return false;
+        const auto key = std::make_tuple(bp.rc, bp.cc, bp.ell);
+        auto it = moves_of.find(key);
+        if (it == moves_of.end()) {
+            ChunkMoves m;
+            *err = chunk_moves_create(bp, &m);
+            if (*err != IPT_OK) return false;
+            b->moves.push_back(m);
+            it = moves_of.emplace(key, b->moves.size() - 1).first;
+        }
+        const ChunkMoves& m = b->moves[it->second];
+        const uint64_t per_slice = chunk_head_bytes(bp, m);
+        if (per_slice > kScratchCap) return false;
+        const uint64_t group =
+            per_slice ? std::max<uint64_t>(1, kScratchCap / per_slice) : (uint64_t)s.d3;
+        head_bytes = std::max<size_t>(head_bytes, std::min<uint64_t>(group, s.d3) * per_slice);
+        *err = add_items(data, bp.blk, rb);
+        if (*err != IPT_OK) return false;
+        const ipt_batch::Chunk c{data, (uint64_t)s.d3, (uint64_t)s.d1 * s.d2 * s.esz / 16, group,
+                                 bp, it->second};
+        (bp.chunk_first ? b->pre : b->post).push_back(c);
+        b->info.n_block_cycle++;
+        return true;
+    };
     for (int64_t i = 0; i < count; i++) {
         const ipt_matrix_desc& d = descs[i];
         const Shape s = make_shape(d.elem_bytes, d.d1, d.d2, d.d3);
@@ -222,78 +337,35 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             continue;
         }
         b->info.bytes += (uint64_t)s.d1 * s.d2 * s.d3 * s.esz;
-        if (p == Route::Big) {
-            if (big_variant == IPT_VARIANT_AUTO) big_variant = choose_variant(ctx, s);
-            // The block-cycle path allocates its segment table on a shape's
-            // first use and stages chunks in scratch, neither of which a
-            // captured batch may do: its instances take the decomposition.
-            if (big_variant == IPT_VARIANT_BLOCK_CYCLE)
-                big_variant = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
-            const bool c2r = big_variant == IPT_VARIANT_C2R;
-            if ((big_variant == IPT_VARIANT_C2R || big_variant == IPT_VARIANT_R2C) &&
-                decomp_needs_scratch(s, c2r, b->tune)) {
+        if (p != Route::Big) {
+            st = add_items(d.data, s, p);
+            if (st != IPT_OK) {
                 release(b);
-                return IPT_ERR_UNSUPPORTED;  // scratch tiers are not used inside a batch
+                return st;
             }
-            if (big_variant == IPT_VARIANT_GCS || !variant_feasible(ctx, s, big_variant)) {
-                release(b);
-                return IPT_ERR_UNSUPPORTED;
-            }
-            b->big.push_back({d.data, s, big_variant, big_tune});
-            b->info.n_big++;
             continue;
         }
-        BatchInst I{};
-        I.data = (uint64_t)(uintptr_t)d.data;
-        I.R = (uint32_t)s.d2;
-        I.C = (uint32_t)s.d1;
-        I.esz = (uint32_t)s.esz;
-        const uint32_t idx = (uint32_t)insts.size();
-        if (p == Route::Slice) {
-            I.kind = BK_SLICE;
-            I.P = (uint32_t)slice_pitch(s.d1);
-            I.dS = Div32::make(I.R * I.C);
-            I.dC = Div32::make(I.C);
-            I.dR = Div32::make(I.R);
-            const uint64_t slice_bytes = (uint64_t)I.R * I.P * s.esz;
-            const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(
-                (uint64_t)b->tune.lds_bytes / slice_bytes, (uint64_t)lds_cap / slice_bytes));
-            lds_need = std::max<size_t>(lds_need, std::min<uint64_t>(per, (uint64_t)s.d3) * slice_bytes);
-            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl += per) {
-                BatchItem it{idx, (uint32_t)std::min<uint64_t>(per, (uint64_t)s.d3 - sl), sl};
-                slice_sorted.push_back({slice_bytes * it.sub, it});
-            }
-            b->info.n_slice_items += ((uint64_t)s.d3 + per - 1) / per;
-        } else if (p == Route::Gcs) {
-            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, ((uintptr_t)d.data % 16) == 0);
-            I.kind = gg.vec ? BK_GCS_VEC : BK_GCS_SCALAR;
-            I.g = gg.g;
-            I.K = gg.K;
-            I.KL = gg.KL;
-            I.ntiles = gg.ntiles;
-            I.dntj = Div32::make(gg.ntj);
-            I.bar = bar_words_total;
-            bar_words_total += bar_words(gg.g) * (uint64_t)s.d3;
-            lds_need = std::max(lds_need, gg.lds);
-            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl++)
-                for (uint32_t m = 0; m < gg.g; m++) gcs_items.push_back({idx, m, sl});
-            b->info.n_gcs_items += (uint64_t)gg.g * s.d3;
-        } else {
-            const uint64_t TS = s.esz == 4 ? 64 : 32;
-            I.kind = ((uint64_t)s.d1 * s.esz) % 16 == 0 && (uintptr_t)d.data % 16 == 0 ? BK_SWAP_VEC
-                                                                                        : BK_SWAP;
-            const uint64_t nt = ((uint64_t)s.d1 + TS - 1) / TS;
-            const uint64_t pairs = nt * (nt + 1) / 2;
-            if (pairs >= ((uint64_t)1 << 32)) {
+        if (big_variant == IPT_VARIANT_AUTO) big_variant = choose_variant(ctx, s);
+        if (big_variant == IPT_VARIANT_BLOCK_CYCLE) {
+            if (add_block_cycle(d.data, s, &st)) continue;
+            if (st != IPT_OK) {
                 release(b);
-                return IPT_ERR_TOO_LARGE;
+                return st;
             }
-            lds_need = std::max<size_t>(lds_need, 2 * TS * (TS + 1) * s.esz);
-            for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl++)
-                for (uint64_t t = 0; t < pairs; t++) swap_items.push_back({idx, (uint32_t)t, sl});
-            b->info.n_swap_items += pairs * s.d3;
+            big_variant = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
         }
-        insts.push_back(I);
+        const bool c2r = big_variant == IPT_VARIANT_C2R;
+        if ((big_variant == IPT_VARIANT_C2R || big_variant == IPT_VARIANT_R2C) &&
+            decomp_needs_scratch(s, c2r, b->tune)) {
+            release(b);
+            return IPT_ERR_UNSUPPORTED;  // scratch tiers are not used inside a batch
+        }
+        if (big_variant == IPT_VARIANT_GCS || !variant_feasible(ctx, s, big_variant)) {
+            release(b);
+            return IPT_ERR_UNSUPPORTED;
+        }
+        b->big.push_back({d.data, s, big_variant, big_tune});
+        b->info.n_big++;
     }
     // Heavy items first; the members of a group stay consecutive.
     std::stable_sort(slice_sorted.begin(), slice_sorted.end(),
@@ -304,6 +376,13 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     items.insert(items.end(), swap_items.begin(), swap_items.end());
     for (const auto& x : slice_sorted) items.push_back(x.second);
 
+    if (!b->pre.empty() || !b->post.empty()) {
+        st = chunk_kernels_setup(ctx);
+        if (st != IPT_OK) {
+            release(b);
+            return st;
+        }
+    }
     if (items.size() >= ((uint64_t)1 << 32)) {  // the work queue counts in 32 bits
         release(b);
         return IPT_ERR_TOO_LARGE;
@@ -326,6 +405,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         e = hipMemcpy(b->d_insts, insts.data(), insts.size() * sizeof(BatchInst), hipMemcpyHostToDevice);
     if (e == hipSuccess && !items.empty())
         e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess && head_bytes) e = hipMalloc(&b->d_head, head_bytes);
     if (e == hipSuccess && b->lds > 64 * 1024)
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds);

@@ -199,64 +199,18 @@ BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s) {
     return p;
 }
 
-// Move plans, built on the host once per (device, rc, cc, ell) and kept in
-// device memory until the process ends.
-struct MovePlan {
-    uint2* items = nullptr;    // (first move, moves)
-    uint32_t* msrc = nullptr;  // source position, or kHeadFlag | head slot
-    uint32_t* mdst = nullptr;  // destination position
-    uint32_t* hpos = nullptr;  // position of each saved head
-    uint32_t nitems = 0, nheads = 0;
-};
-static std::mutex g_plan_mu;
-static std::map<std::tuple<int, int64_t, int64_t, int64_t>, MovePlan> g_plans;
-static size_t g_plan_bytes = 0;  // device bytes held by g_plans
-
-// Past kBcPlanCacheBytes the cache is emptied (after a device synchronise:
-// queued kernels may still read the plans) before the next plan is added.
-static hipError_t drop_plans_locked() {
-    const hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) return e;
-    for (auto& kv : g_plans) {
-        (void)hipFree(kv.second.items);
-        (void)hipFree(kv.second.msrc);
-        (void)hipFree(kv.second.mdst);
-        (void)hipFree(kv.second.hpos);
-    }
-    g_plans.clear();
-    g_plan_bytes = 0;
-    return hipSuccess;
-}
-
-template <typename V>
-static hipError_t upload(const std::vector<V>& h, V** d) {
-    *d = nullptr;
-    if (h.empty()) return hipSuccess;
-    hipError_t e = hipMalloc(d, h.size() * sizeof(V));
-    if (e == hipSuccess) e = hipMemcpy(*d, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice);
-    return e;
-}
-
-static ipt_status move_plan(const DeviceCtx* ctx, const BlockCyclePlan& p, hipStream_t st,
-                            MovePlan* out) {
-    std::lock_guard<std::mutex> g(g_plan_mu);
-    const auto key = std::make_tuple(ctx->dev, p.rc, p.cc, p.ell);
-    auto it = g_plans.find(key);
-    if (it != g_plans.end()) {
-        *out = it->second;
-        return IPT_OK;
-    }
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
-        return IPT_ERR_UNSUPPORTED;  // building a plan allocates
+// Move lists of the chunk permutation, built on the host.  Work items in
+// cycle order; cut segments' heads in hpos.
+static void build_moves(const BlockCyclePlan& p, std::vector<uint2>* items,
+                        std::vector<uint32_t>* msrc, std::vector<uint32_t>* mdst,
+                        std::vector<uint32_t>* hpos) {
     const uint32_t N = (uint32_t)(p.rc * p.cc), rc = (uint32_t)p.rc, cc = (uint32_t)p.cc;
     const uint32_t ell = (uint32_t)p.ell;
     std::vector<uint8_t> seen(N, 0);
-    std::vector<uint2> items;
-    std::vector<uint32_t> msrc, mdst, hpos, cyc;
+    std::vector<uint32_t> cyc;
     uint32_t bundle = 0;  // moves in the open item of whole short cycles
     auto close_bundle = [&] {
-        if (bundle) items.push_back(make_uint2((uint32_t)msrc.size() - bundle, bundle));
+        if (bundle) items->push_back(make_uint2((uint32_t)msrc->size() - bundle, bundle));
         bundle = 0;
     };
     for (uint32_t q0 = 0; q0 < N; q0++) {
@@ -273,8 +227,8 @@ static ipt_status move_plan(const DeviceCtx* ctx, const BlockCyclePlan& p, hipSt
         if (n <= ell) {
             if (bundle + n > ell) close_bundle();
             for (uint32_t j = 0; j < n; j++) {
-                msrc.push_back(cyc[j]);
-                mdst.push_back(cyc[(j + 1) % n]);
+                msrc->push_back(cyc[j]);
+                mdst->push_back(cyc[(j + 1) % n]);
             }
             bundle += n;
             continue;
@@ -282,38 +236,90 @@ static ipt_status move_plan(const DeviceCtx* ctx, const BlockCyclePlan& p, hipSt
         close_bundle();
         for (uint32_t a = 0; a < n; a += ell) {
             const uint32_t len = std::min(ell, n - a);
-            items.push_back(make_uint2((uint32_t)msrc.size(), len));
+            items->push_back(make_uint2((uint32_t)msrc->size(), len));
             for (uint32_t j = 0; j < len; j++) {
-                msrc.push_back(j == 0 ? (kHeadFlag | (uint32_t)hpos.size()) : cyc[a + j]);
-                mdst.push_back(cyc[(a + j + 1) % n]);
+                msrc->push_back(j == 0 ? (kHeadFlag | (uint32_t)hpos->size()) : cyc[a + j]);
+                mdst->push_back(cyc[(a + j + 1) % n]);
             }
-            hpos.push_back(cyc[a]);
+            hpos->push_back(cyc[a]);
         }
     }
     close_bundle();
-    const size_t bytes = items.size() * sizeof(uint2) + (msrc.size() + mdst.size() + hpos.size()) * 4;
-    if (g_plan_bytes + bytes > kBcPlanCacheBytes && !g_plans.empty()) {
-        const hipError_t e = drop_plans_locked();
-        if (e != hipSuccess) {
-            set_error("hipDeviceSynchronize", e);
-            return IPT_ERR_HIP;
-        }
-    }
-    MovePlan m;
+}
+
+template <typename V>
+static hipError_t upload(const std::vector<V>& h, V** d) {
+    *d = nullptr;
+    if (h.empty()) return hipSuccess;
+    hipError_t e = hipMalloc(d, h.size() * sizeof(V));
+    if (e == hipSuccess) e = hipMemcpy(*d, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice);
+    return e;
+}
+
+void chunk_moves_free(ChunkMoves* m) {
+    (void)hipFree(m->items);
+    (void)hipFree(m->msrc);
+    (void)hipFree(m->mdst);
+    (void)hipFree(m->hpos);
+    *m = ChunkMoves{};
+}
+
+ipt_status chunk_moves_create(const BlockCyclePlan& p, ChunkMoves* out) {
+    std::vector<uint2> items;
+    std::vector<uint32_t> msrc, mdst, hpos;
+    build_moves(p, &items, &msrc, &mdst, &hpos);
+    ChunkMoves m;
     m.nitems = (uint32_t)items.size();
     m.nheads = (uint32_t)hpos.size();
+    m.bytes = items.size() * sizeof(uint2) + (msrc.size() + mdst.size() + hpos.size()) * 4;
     hipError_t e = upload(items, &m.items);
     if (e == hipSuccess) e = upload(msrc, &m.msrc);
     if (e == hipSuccess) e = upload(mdst, &m.mdst);
     if (e == hipSuccess) e = upload(hpos, &m.hpos);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        (void)hipFree(m.items);
-        (void)hipFree(m.msrc);
-        (void)hipFree(m.mdst);
-        (void)hipFree(m.hpos);
+        chunk_moves_free(&m);
         set_error("move plan upload", e);
         return IPT_ERR_OUT_OF_MEMORY;
+    }
+    *out = m;
+    return IPT_OK;
+}
+
+// Move lists used by the per-call path, built once per (device, rc, cc, ell)
+// and kept in device memory; past kBcPlanCacheBytes the cache is emptied
+// (after a device synchronise: queued kernels may still read the lists)
+// before the next one is added.
+static std::mutex g_plan_mu;
+static std::map<std::tuple<int, int64_t, int64_t, int64_t>, ChunkMoves> g_plans;
+static size_t g_plan_bytes = 0;  // device bytes held by g_plans
+
+static ipt_status cached_moves(const DeviceCtx* ctx, const BlockCyclePlan& p, hipStream_t st,
+                               ChunkMoves* out) {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    const auto key = std::make_tuple(ctx->dev, p.rc, p.cc, p.ell);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+        *out = it->second;
+        return IPT_OK;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return IPT_ERR_UNSUPPORTED;  // building the lists allocates
+    ChunkMoves m;
+    const ipt_status stt = chunk_moves_create(p, &m);
+    if (stt != IPT_OK) return stt;
+    const size_t bytes = m.bytes;
+    if (g_plan_bytes + bytes > kBcPlanCacheBytes && !g_plans.empty()) {
+        const hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            chunk_moves_free(&m);
+            set_error("hipDeviceSynchronize", e);
+            return IPT_ERR_HIP;
+        }
+        for (auto& kv : g_plans) chunk_moves_free(&kv.second);
+        g_plans.clear();
+        g_plan_bytes = 0;
     }
     g_plans[key] = m;
     g_plan_bytes += bytes;
@@ -321,33 +327,61 @@ static ipt_status move_plan(const DeviceCtx* ctx, const BlockCyclePlan& p, hipSt
     return IPT_OK;
 }
 
-// The chunk step over slices [0, d3): saves the heads of cut cycles, then
-// runs the items, in groups of slices whose saved heads fit the scratch cap.
+ipt_status chunk_kernels_setup(const DeviceCtx* ctx) {
+    // staging (up to kBcStageBytes) plus the static move lists exceed the
+    // default dynamic-LDS limit
+    static std::set<int> raised;
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    if (!raised.count(ctx->dev)) {
+        IPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chunk_cycle),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)kBcStageBytes));
+        raised.insert(ctx->dev);
+    }
+    return IPT_OK;
+}
+
+ipt_status chunk_step_launch(const DeviceCtx* ctx, void* data, uint64_t nslices,
+                             uint64_t slice_vecs, const BlockCyclePlan& p, const ChunkMoves& m,
+                             void* head, uint64_t group, hipStream_t st) {
+    if (m.nitems == 0) return IPT_OK;
+    const size_t lds = (size_t)p.ell * p.cv * 16;
+    const Div32 dcv = Div32::make((uint32_t)p.cv);
+    for (uint64_t s0 = 0; s0 < nslices; s0 += group) {
+        const uint64_t ns = std::min<uint64_t>(group, nslices - s0);
+        uint4* d = static_cast<uint4*>(data) + s0 * slice_vecs;
+        if (m.nheads) {
+            const uint64_t save_vecs = ns * m.nheads * (uint64_t)p.cv;
+            const int gs = (int)std::min<uint64_t>((save_vecs + kChunkBlock * 4 - 1) / (kChunkBlock * 4),
+                                                   (uint64_t)ncu_of(ctx) * 8);
+            hipLaunchKernelGGL(k_chunk_save, dim3(gs), dim3(kChunkBlock), 0, st, (const uint4*)d,
+                               (uint4*)head, (const uint32_t*)m.hpos, m.nheads, ns, slice_vecs,
+                               (uint32_t)p.cv, dcv);
+            IPT_LAUNCH_CHECK("k_chunk_save");
+        }
+        const int gc = (int)std::min<uint64_t>(ns * m.nitems, (uint64_t)ncu_of(ctx) * 8);
+        hipLaunchKernelGGL(k_chunk_cycle, dim3(gc), dim3(kChunkBlock), lds, st, d, (const uint4*)head,
+                           (const uint2*)m.items, (const uint32_t*)m.msrc, (const uint32_t*)m.mdst,
+                           m.nitems, m.nheads, ns, slice_vecs, (uint32_t)p.cv, dcv);
+        IPT_LAUNCH_CHECK("k_chunk_cycle");
+    }
+    return IPT_OK;
+}
+
+// The chunk step of the per-call path: cached move lists, heads saved in the
+// scratch arena, in groups of slices whose heads fit the scratch cap.
 static ipt_status chunk_step(DeviceCtx* ctx, void* data, const Shape& s, const BlockCyclePlan& p,
                              hipStream_t st) {
-    MovePlan mp;
-    ipt_status stt = move_plan(ctx, p, st, &mp);
+    ChunkMoves m;
+    ipt_status stt = cached_moves(ctx, p, st, &m);
     if (stt != IPT_OK) return stt;
-    if (mp.nitems == 0) return IPT_OK;
-    const uint64_t chunk_bytes = (uint64_t)p.cv * 16;
-    const uint64_t heads_per_slice = (uint64_t)mp.nheads * chunk_bytes;
+    if (m.nitems == 0) return IPT_OK;
+    const uint64_t heads_per_slice = chunk_head_bytes(p, m);
     if (heads_per_slice > kScratchCap) return IPT_ERR_UNSUPPORTED;
     const uint64_t group =
         heads_per_slice ? std::max<uint64_t>(1, kScratchCap / heads_per_slice) : (uint64_t)s.d3;
-    const uint64_t slice_vecs = (uint64_t)s.d1 * s.d2 * s.esz / 16;
-    const size_t lds = (size_t)p.ell * chunk_bytes;
-    {
-        // staging (up to kBcStageBytes) plus the static move lists exceed the
-        // default dynamic-LDS limit: raise it once per device
-        static std::set<int> raised;
-        std::lock_guard<std::mutex> g(g_plan_mu);
-        if (!raised.count(ctx->dev)) {
-            IPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chunk_cycle),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)kBcStageBytes));
-            raised.insert(ctx->dev);
-        }
-    }
+    stt = chunk_kernels_setup(ctx);
+    if (stt != IPT_OK) return stt;
     std::lock_guard<std::mutex> g(ctx->mu);
     void* head = nullptr;
     if (heads_per_slice) {
@@ -356,27 +390,11 @@ static ipt_status chunk_step(DeviceCtx* ctx, void* data, const Shape& s, const B
                             &head);
         if (stt != IPT_OK) return stt;
     }
-    const Div32 dcv = Div32::make((uint32_t)p.cv);
-    for (uint64_t s0 = 0; s0 < (uint64_t)s.d3; s0 += group) {
-        const uint64_t ns = std::min<uint64_t>(group, (uint64_t)s.d3 - s0);
-        uint4* d = static_cast<uint4*>(data) + s0 * slice_vecs;
-        if (mp.nheads) {
-            const uint64_t save_vecs = ns * mp.nheads * (uint64_t)p.cv;
-            const int gs = (int)std::min<uint64_t>((save_vecs + kChunkBlock * 4 - 1) / (kChunkBlock * 4),
-                                                   (uint64_t)ncu_of(ctx) * 8);
-            hipLaunchKernelGGL(k_chunk_save, dim3(gs), dim3(kChunkBlock), 0, st, (const uint4*)d,
-                               (uint4*)head, (const uint32_t*)mp.hpos, mp.nheads, ns, slice_vecs,
-                               (uint32_t)p.cv, dcv);
-            IPT_LAUNCH_CHECK("k_chunk_save");
-        }
-        const int gc = (int)std::min<uint64_t>(ns * mp.nitems, (uint64_t)ncu_of(ctx) * 8);
-        hipLaunchKernelGGL(k_chunk_cycle, dim3(gc), dim3(kChunkBlock), lds, st, d, (const uint4*)head,
-                           (const uint2*)mp.items, (const uint32_t*)mp.msrc,
-                           (const uint32_t*)mp.mdst, mp.nitems, mp.nheads, ns, slice_vecs,
-                           (uint32_t)p.cv, dcv);
-        IPT_LAUNCH_CHECK("k_chunk_cycle");
-    }
-    return heads_per_slice ? arena_release(ctx->scratch, st) : IPT_OK;
+    const uint64_t slice_vecs = (uint64_t)s.d1 * s.d2 * s.esz / 16;
+    stt = chunk_step_launch(ctx, data, (uint64_t)s.d3, slice_vecs, p, m, head, group, st);
+    if (!heads_per_slice) return stt;
+    const ipt_status r = arena_release(ctx->scratch, st);
+    return stt != IPT_OK ? stt : r;
 }
 
 ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,

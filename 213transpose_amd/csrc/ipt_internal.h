@@ -153,6 +153,32 @@ BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s);
 ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,
                            hipStream_t st);
 
+// The chunk step's move lists in device memory (see ipt_block_cycle.hip).
+struct ChunkMoves {
+    uint2* items = nullptr;    // (first move, moves)
+    uint32_t* msrc = nullptr;  // source position, or a flagged head slot
+    uint32_t* mdst = nullptr;  // destination position
+    uint32_t* hpos = nullptr;  // position of each saved head
+    uint32_t nitems = 0, nheads = 0;
+    size_t bytes = 0;  // device bytes held
+};
+// Builds and uploads the move lists of plan p; the caller owns them and
+// frees them with chunk_moves_free.  Allocates: never under capture.
+ipt_status chunk_moves_create(const BlockCyclePlan& p, ChunkMoves* out);
+void chunk_moves_free(ChunkMoves* m);
+// Raises the chunk kernel's dynamic-LDS limit on this device (once).
+ipt_status chunk_kernels_setup(const DeviceCtx* ctx);
+// Saved-head bytes per slice.
+inline uint64_t chunk_head_bytes(const BlockCyclePlan& p, const ChunkMoves& m) {
+    return (uint64_t)m.nheads * (uint64_t)p.cv * 16;
+}
+// Chunk step over `nslices` slices of `slice_vecs` 16-byte vectors at data,
+// `group` slices at a time through `head` (group * chunk_head_bytes bytes).
+// Launches only: nothing is allocated, so it may be captured.
+ipt_status chunk_step_launch(const DeviceCtx* ctx, void* data, uint64_t nslices,
+                             uint64_t slice_vecs, const BlockCyclePlan& p, const ChunkMoves& m,
+                             void* head, uint64_t group, hipStream_t st);
+
 template <typename Ix>
 ipt_status eval_map_host(int map_id, const HostPlan& h, const int64_t* i, const int64_t* j,
                          int64_t n, int64_t* out) {

@@ -231,8 +231,12 @@ int ipt_select_candidate(int elem_bytes, int64_t d1, int64_t d2, int64_t d3);
 /* No counterpart in the reference (one tensor per call, src/transpose.cu:
  * 99-124).  A batch is planned once: every instance becomes work items of one
  * persistent kernel (whole-slice LDS, cooperative group, or square tile-pair
- * items); instances too large for those run through the per-call C2R / R2C
- * launchers (only their LDS tiers: a batch never uses library scratch).
+ * items).  An instance too large for those takes the block-cycle path when
+ * it can: its blocks become items of the persistent kernel, and its chunk
+ * steps run just before (chunked columns) or after (chunked rows) that kernel,
+ * on move lists and a head buffer the batch owns.  The rest run through the
+ * per-call C2R / R2C launchers (only their LDS tiers: a batch never uses
+ * library scratch).
  * ipt_batch_run captures everything once into a hipGraph on the first call
  * and replays it afterwards.  Instance buffers must stay valid and must not
  * overlap; a batch must not run concurrently with itself. */
@@ -248,6 +252,7 @@ typedef struct {
     int64_t n_slice_items, n_gcs_items, n_swap_items;
     int64_t grid, lds_bytes;
     uint64_t bytes;      /* payload bytes of the transposed instances */
+    int64_t n_block_cycle; /* instances on the block-cycle path */
 } ipt_batch_info;
 
 typedef struct ipt_batch_s ipt_batch;

@@ -92,6 +92,36 @@ def test_batch_forced_variants_and_small_lds(ipt, oracle):
 
 
 @pytest.mark.gpu
+def test_batch_block_cycle_instances(ipt, oracle):
+    """Instances on the block-cycle path inside a batch: chunk steps before
+    (chunked columns) and after (chunked rows) the persistent kernel, blocks as
+    its items; a shape the path cannot take falls back to C2R / R2C."""
+    BC, AUTO = ipt.VARIANT_BLOCK_CYCLE, ipt.VARIANT_AUTO
+    cases = [((65536, 8, 2, 4), BC), ((8, 65536, 1, 8), BC), ((6144, 2048, 1, 4), BC),
+             ((2048, 6144, 1, 4), BC), ((4096, 96, 3, 8), BC), ((96, 4096, 2, 4), AUTO),
+             ((1001, 3, 1, 4), BC), ((300, 200, 1, 4), AUTO)]
+    buf, items, host = make_instances(oracle, [c for c, _ in cases], 1200)
+    b = ipt.Batch([it + (v,) for it, (_, v) in zip(items, cases)])
+    info = b.info()
+    print("[batch block-cycle]", info, flush=True)
+    assert info["n_block_cycle"] >= 5 and info["n_big"] >= 1
+    b.run(graph=False)
+    torch.cuda.synchronize()
+    check(oracle, items, host, 1)
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    check(oracle, items, host, 2)
+    scratch = torch.zeros(1 << 20, device="cuda")
+    scratch += 1
+    b.run(graph=True)
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, items, host, 4)
+    b.close()
+
+
+@pytest.mark.gpu
 def test_uneven_workload_sample(ipt, oracle):
     """A 300-instance sample of the uneven-batch workload (m, n log-uniform in
     [2, 1024]), one graph launch, every instance checked."""
