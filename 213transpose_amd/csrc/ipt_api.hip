@@ -55,6 +55,12 @@ ipt_status get_ctx(DeviceCtx** out) {
         return IPT_ERR_NO_DEVICE;
     }
     c->ncu = prop.multiProcessorCount;
+    int nxcd = 1;
+    if (hipDeviceGetAttribute(&nxcd, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || nxcd < 1) {
+        (void)hipGetLastError();
+        nxcd = 1;
+    }
+    c->nxcd = nxcd;
     c->sync.zero_at_rest = true;
     c->lds_max = (int64_t)prop.sharedMemPerBlock;
     snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);

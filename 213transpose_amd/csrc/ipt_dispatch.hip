@@ -246,7 +246,7 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
 // Signature shared by both GCS kernels (k_gcs2: vector path, k_gcs: scalar).
 template <typename T>
 using GcsKernel = void (*)(T*, uint32_t, uint32_t, uint64_t, uint32_t, uint32_t, uint32_t, Div32,
-                           uint32_t, uint32_t*, uint32_t*);
+                           uint32_t, uint32_t*, uint32_t*, uint32_t);
 
 template <typename T>
 static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& s,
@@ -263,7 +263,7 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
     if (gg.g == 1) {  // whole slices per workgroup: no barrier state needed
         hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, d, (uint32_t)s.d2,
                            (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL, Div32::make(gg.ntj),
-                           gg.ntiles, (uint32_t*)nullptr, (uint32_t*)nullptr);
+                           gg.ntiles, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t)ctx->nxcd);
         IPT_LAUNCH_CHECK("k_gcs");
         return IPT_OK;
     }
@@ -282,13 +282,13 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
     if (ev) {
         hipExtLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, nullptr, ev, 0, d,
                               (uint32_t)s.d2, (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL,
-                              Div32::make(gg.ntj), gg.ntiles, ctl, ctl + 32);
+                              Div32::make(gg.ntj), gg.ntiles, ctl, ctl + 32, (uint32_t)ctx->nxcd);
         IPT_LAUNCH_CHECK("k_gcs");
         return IPT_OK;
     }
     hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, d, (uint32_t)s.d2,
                        (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL, Div32::make(gg.ntj),
-                       gg.ntiles, ctl, ctl + 32);
+                       gg.ntiles, ctl, ctl + 32, (uint32_t)ctx->nxcd);
     IPT_LAUNCH_CHECK("k_gcs");
     return arena_release(ctx->sync, st);
 }

@@ -30,6 +30,7 @@ struct Arena {
 struct DeviceCtx {
     int dev = 0;
     int ncu = 0;
+    int nxcd = 1;  // accelerator complex dies (XCDs); workgroup i runs on XCD i % nxcd
     int64_t lds_max = 0;
     char arch[64] = {0};
     std::mutex mu;  // guards the arenas and serialises enqueues that use them

@@ -656,6 +656,21 @@ k_square_swap_rs(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3) 
     }
 }
 
+// Slot numbering for the cooperative kernels.  Workgroups are dispatched
+// round-robin over the device's XCDs (workgroup i on XCD i % nxcd), and each
+// XCD has its own L2.  A member that writes lines another member read just
+// before is about twice as slow when that member ran on another XCD
+// (measured on the [512][384][64] f32 permute: 24.1 -> 16.5 us per launch
+// with XCD-local groups), so slots are numbered to give each XCD one
+// contiguous range: consecutive slots, the members of one slice, then share
+// an L2.  A bijection on [0, n): the set of slots run, and the deadlock-
+// freedom argument (which is over slot order), do not depend on it.
+__device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n, uint32_t nxcd) {
+    if (nxcd <= 1) return i;
+    const uint32_t q = n / nxcd, rem = n % nxcd, x = i % nxcd;
+    return x * q + min(x, rem) + i / nxcd;
+}
+
 // Wave tiles a wave stages in LDS (JL) and holds in registers (JR) on the
 // vector path, per element size: 40 / 80 LDS tiles fill 160 KB, and the
 // register tiles hold 64 KB per 512-thread workgroup.
@@ -737,16 +752,17 @@ __device__ __forceinline__ void gcs2_body(T* __restrict__ base, uint32_t R, uint
     if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
 }
 
-// Vector-path GCS kernel: slot s is member s % g of slice s / g.
+// Vector-path GCS kernel: slot s (numbered by xcd_slot) is member s % g of
+// slice s / g.
 template <typename T, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_gcs2(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
        uint32_t KL, Div32 dntj, uint32_t ntiles, uint32_t* __restrict__ ctl,
-       uint32_t* __restrict__ cnt) {
+       uint32_t* __restrict__ cnt, uint32_t nxcd) {
     IPT_DYN_LDS(u32x4, lds);
     const uint64_t total = nslices * g;
     const uint64_t slice_elems = (uint64_t)R * C;
-    for (uint64_t slot = blockIdx.x; slot < total; slot += gridDim.x) {
+    for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd); slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
         gcs2_body<T, BLOCK>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
@@ -912,11 +928,11 @@ template <typename T, int TS, int BLOCK, int RV>
 __global__ void __launch_bounds__(BLOCK)
 k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
       uint32_t KL, Div32 dntj, uint32_t ntiles, uint32_t* __restrict__ ctl,
-      uint32_t* __restrict__ cnt) {
+      uint32_t* __restrict__ cnt, uint32_t nxcd) {
     IPT_DYN_LDS(T, lds);
     const uint64_t total = nslices * g;
     const uint64_t slice_elems = (uint64_t)R * C;
-    for (uint64_t slot = blockIdx.x; slot < total; slot += gridDim.x) {
+    for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd); slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
         gcs_body<T, TS, BLOCK, RV>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
