@@ -18,33 +18,59 @@
 
 namespace ipt {
 
-// Work queue: a workgroup takes the next item (ctl[kQueueNext]) whenever it
-// is free, so items of uneven cost balance across the grid.  ctl[kQueueDone]
-// counts workgroups that found the queue empty; the last one zeroes both
-// words (zero at rest, so graph replays start from an empty count).
-constexpr int kQueueNext = 8, kQueueDone = 9;
+// Work queues.  Items sit in nq + 2 segments of one table, seg[q] .. seg[q+1]:
+//   queue 0       groups of the cooperative kernel too wide for one XCD's
+//                 share of the grid (taken first, by every workgroup);
+//   queue 1 + x   groups placed on XCD x (workgroup i runs on XCD i % nq):
+//                 the members of a group then share an L2, which halves the
+//                 cost of their stores (each writes lines another just read);
+//   queue nq + 1  items with no traffic between workgroups (whole slices,
+//                 tile pairs), shared by all.
+// A workgroup drains queue 0, its own XCD's queue, the shared queue, then
+// the other XCDs' queues.  Taking from a queue is one atomic on its counter
+// ctl[kQueueBase + q], so items of uneven cost balance across the grid.
+// Deadlock freedom: within a queue items are taken in order, so only its
+// head group can be partly taken; once any workgroup has left queue 0 every
+// group there is fully taken (and completes).  A stuck state needs every
+// workgroup waiting in a partly taken group, at most one per XCD queue, each
+// with at most g - 1 members: impossible while nq * (g - 1) < grid, which the
+// host guarantees for every group it places in an XCD queue (queue 0 needs
+// only g <= grid).  ctl[kQueueDone] counts finished workgroups; the last one
+// zeroes the counters (zero at rest, so graph replays start empty).
+constexpr int kQueueDone = 9, kQueueBase = 16;
+constexpr uint32_t kMaxXcdQueues = 8;
 
 __global__ void __launch_bounds__(kBatchBlock)
-k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items, uint32_t nitems,
-        uint32_t* __restrict__ ctl, uint32_t* __restrict__ bars) {
+k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items,
+        const uint32_t* __restrict__ seg, uint32_t nq, uint32_t* __restrict__ ctl,
+        uint32_t* __restrict__ bars) {
     IPT_DYN_LDS(unsigned char, lds);
     __shared__ uint32_t next;
-    for (;;) {
-        if (threadIdx.x == 0) next = atomic_add_agent(&ctl[kQueueNext], 1u);
-        __syncthreads();
-        const uint32_t s = next;
-        __syncthreads();  // `next` is rewritten by the next take
-        if (s >= nitems) break;
-        const BatchItem it = items[s];
-        const BatchInst I = insts[it.inst];
-        if (I.esz == 4)
-            batch_item<uint32_t>(I, it, ctl, bars, reinterpret_cast<uint32_t*>(lds));
-        else
-            batch_item<uint64_t>(I, it, ctl, bars, reinterpret_cast<uint64_t*>(lds));
-        __syncthreads();  // LDS is reused by the next item
+    const uint32_t own = blockIdx.x % nq;
+    for (uint32_t step = 0; step < nq + 2; step++) {
+        const uint32_t q = step == 0 ? 0u
+                         : step == 1 ? 1 + own
+                         : step == 2 ? nq + 1
+                                     : 1 + (own + step - 2) % nq;
+        const uint32_t lo = seg[q], hi = seg[q + 1];
+        if (lo == hi) continue;
+        for (;;) {
+            if (threadIdx.x == 0) next = lo + atomic_add_agent(&ctl[kQueueBase + q], 1u);
+            __syncthreads();
+            const uint32_t s = next;
+            __syncthreads();  // `next` is rewritten by the next take
+            if (s >= hi) break;
+            const BatchItem it = items[s];
+            const BatchInst I = insts[it.inst];
+            if (I.esz == 4)
+                batch_item<uint32_t>(I, it, ctl, bars, reinterpret_cast<uint32_t*>(lds));
+            else
+                batch_item<uint64_t>(I, it, ctl, bars, reinterpret_cast<uint64_t*>(lds));
+            __syncthreads();  // LDS is reused by the next item
+        }
     }
     if (threadIdx.x == 0 && atomic_add_agent(&ctl[kQueueDone], 1u) == gridDim.x - 1) {
-        atomic_store_agent(&ctl[kQueueNext], 0u);
+        for (uint32_t q = 0; q < nq + 2; q++) atomic_store_agent(&ctl[kQueueBase + q], 0u);
         atomic_store_agent(&ctl[kQueueDone], 0u);
     }
 }
@@ -57,6 +83,8 @@ struct ipt_batch_s {
     DeviceCtx* ctx = nullptr;
     BatchInst* d_insts = nullptr;
     BatchItem* d_items = nullptr;
+    uint32_t* d_seg = nullptr;  // queue segments of d_items (nq + 3 words)
+    uint32_t nq = 1;            // XCD queues
     uint32_t* d_sync = nullptr;  // 32 control words + barrier state (zero at rest)
     uint64_t sync_words = 0;
     uint64_t nitems = 0;
@@ -153,7 +181,7 @@ ipt_status enqueue(ipt_batch* b, hipStream_t st) {
     if (r != IPT_OK) return r;
     if (b->nitems) {
         hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBatchBlock), b->lds, st, b->d_insts,
-                           b->d_items, (uint32_t)b->nitems, b->d_sync, b->d_sync + 32);
+                           b->d_items, (const uint32_t*)b->d_seg, b->nq, b->d_sync, b->d_sync + 32);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             set_error("k_batch", e);
@@ -175,6 +203,7 @@ void release(ipt_batch* b) {
     if (b->cap) (void)hipStreamDestroy(b->cap);
     if (b->d_insts) (void)hipFree(b->d_insts);
     if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_seg) (void)hipFree(b->d_seg);
     if (b->d_sync) (void)hipFree(b->d_sync);
     for (auto& m : b->moves) chunk_moves_free(&m);
     if (b->d_head) (void)hipFree(b->d_head);
@@ -224,7 +253,13 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     size_t next_auto = 0;
 
     std::vector<BatchInst> insts;
-    std::vector<BatchItem> gcs_items, swap_items, slice_items;
+    // Cooperative groups (one slice of an instance: g consecutive items).
+    struct Group {
+        uint32_t inst, g;
+        uint64_t sl, bytes;
+    };
+    std::vector<Group> groups;
+    std::vector<BatchItem> swap_items;
     std::vector<std::pair<uint64_t, BatchItem>> slice_sorted;
     uint64_t bar_words_total = 0;
     size_t lds_need = 0;
@@ -252,8 +287,9 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             }
             b->info.n_slice_items += ((uint64_t)s.d3 + per - 1) / per;
         } else if (p == Route::Gcs) {
-            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, ((uintptr_t)data % 16) == 0);
-            I.kind = gg.vec ? BK_GCS_VEC : BK_GCS_SCALAR;
+            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false);
+            I.kind = BK_GCS;
+            I.jr = (uint32_t)gcs_jr_class(s.esz, gg.K, gg.KL, kBatchBlock / 64);
             I.g = gg.g;
             I.K = gg.K;
             I.KL = gg.KL;
@@ -263,7 +299,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             bar_words_total += bar_words(gg.g) * (uint64_t)s.d3;
             lds_need = std::max(lds_need, gg.lds);
             for (uint64_t sl = 0; sl < (uint64_t)s.d3; sl++)
-                for (uint32_t m = 0; m < gg.g; m++) gcs_items.push_back({idx, m, sl});
+                groups.push_back({idx, gg.g, sl, (uint64_t)s.d1 * s.d2 * s.esz});
             b->info.n_gcs_items += (uint64_t)gg.g * s.d3;
         } else {
             const uint64_t TS = s.esz == 4 ? 64 : 32;
@@ -367,14 +403,40 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         b->big.push_back({d.data, s, big_variant, big_tune});
         b->info.n_big++;
     }
-    // Heavy items first; the members of a group stay consecutive.
+    // Queues (see k_batch).  Heavy work first everywhere; the members of a
+    // group stay consecutive.  Groups go to the least loaded XCD queue, in
+    // decreasing size, when nq * (g - 1) < grid; the rest to queue 0.
+    uint64_t n_gcs_items = 0;
+    for (const Group& x : groups) n_gcs_items += x.g;
+    const uint64_t nitems_total = n_gcs_items + swap_items.size() + slice_sorted.size();
+    const int grid_final = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)grid, nitems_total));
+    uint32_t nq = (uint32_t)std::min<int>(std::max(ctx->nxcd, 1), (int)kMaxXcdQueues);
+    if ((uint32_t)grid_final < 2 * nq) nq = 1;
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const Group& x, const Group& y) { return x.bytes > y.bytes; });
     std::stable_sort(slice_sorted.begin(), slice_sorted.end(),
                      [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<std::vector<BatchItem>> queues(nq + 2);
+    std::vector<uint64_t> load(nq, 0);
+    for (const Group& x : groups) {
+        size_t q = 0;
+        if ((uint64_t)nq * (x.g - 1) < (uint64_t)grid_final) {
+            const size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[k] += x.bytes;
+            q = 1 + k;
+        }
+        for (uint32_t m = 0; m < x.g; m++) queues[q].push_back({x.inst, m, x.sl});
+    }
+    queues[nq + 1] = swap_items;
+    for (const auto& x : slice_sorted) queues[nq + 1].push_back(x.second);
     std::vector<BatchItem> items;
-    items.reserve(gcs_items.size() + swap_items.size() + slice_sorted.size());
-    items.insert(items.end(), gcs_items.begin(), gcs_items.end());
-    items.insert(items.end(), swap_items.begin(), swap_items.end());
-    for (const auto& x : slice_sorted) items.push_back(x.second);
+    std::vector<uint32_t> seg(nq + 3, 0);
+    items.reserve(nitems_total);
+    for (uint32_t q = 0; q < nq + 2; q++) {
+        seg[q] = (uint32_t)items.size();
+        items.insert(items.end(), queues[q].begin(), queues[q].end());
+    }
+    seg[nq + 2] = (uint32_t)items.size();
 
     if (!b->pre.empty() || !b->post.empty()) {
         st = chunk_kernels_setup(ctx);
@@ -383,13 +445,16 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             return st;
         }
     }
-    if (items.size() >= ((uint64_t)1 << 32)) {  // the work queue counts in 32 bits
+    // queue positions count in 32 bits, and each counter overshoots by at
+    // most one take per workgroup
+    if (nitems_total + (uint64_t)grid_final >= ((uint64_t)1 << 32)) {
         release(b);
         return IPT_ERR_TOO_LARGE;
     }
     b->nitems = items.size();
+    b->nq = nq;
     b->lds = std::max<size_t>(lds_need, 16);
-    b->grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)grid, b->nitems));
+    b->grid = grid_final;
     b->info.count = count;
     b->info.n_instances = (int64_t)insts.size();
     b->info.grid = b->grid;
@@ -397,6 +462,9 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     hipError_t e = hipSuccess;
     if (!insts.empty()) e = hipMalloc(&b->d_insts, insts.size() * sizeof(BatchInst));
     if (e == hipSuccess && !items.empty()) e = hipMalloc(&b->d_items, items.size() * sizeof(BatchItem));
+    if (e == hipSuccess) e = hipMalloc(&b->d_seg, seg.size() * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_seg, seg.data(), seg.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     b->sync_words = 32 + bar_words_total;
     const size_t sync_bytes = b->sync_words * sizeof(uint32_t);
     if (e == hipSuccess) e = hipMalloc(&b->d_sync, sync_bytes);

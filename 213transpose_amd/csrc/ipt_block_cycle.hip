@@ -141,16 +141,15 @@ k_chunk_cycle(uint4* __restrict__ data, const uint4* __restrict__ head, const ui
 //               extra (short contiguous runs per position);
 //   block step  1 / efficiency of its kernel relative to a copy (measured:
 //               square swap ~1, whole-slice 16-byte form ~0.9, cooperative
-//               vector path ~0.85, cooperative scalar path ~0.5 or less).
+//               kernel ~0.85, whole-slice scalar form ~0.5).
 // The cheapest wins; ties go to the larger chunk.
 static int block_kernel(const DeviceCtx* ctx, const Shape& blk, double* eff) {
     const int64_t bytes = blk.d1 * blk.d2 * blk.esz;
     if (blk.d1 == blk.d2) { *eff = 1.0; return IPT_VARIANT_SQUARE_SWAP; }
-    const GcsGeom gg = gcs_geom(ctx, blk, lds_max_of(ctx), true, true);
+    const GcsGeom gg = gcs_geom(ctx, blk, lds_max_of(ctx), true);
     const bool slice = bytes <= kBcLdsBlockBytes && variant_feasible(ctx, blk, IPT_VARIANT_SLICE_LDS);
     if (slice && (bytes % 16) == 0) { *eff = 0.9; return IPT_VARIANT_SLICE_LDS; }
-    if (gg.ok && gg.vec) { *eff = 0.85; return IPT_VARIANT_GCS; }
-    if (gg.ok) { *eff = 0.5; return IPT_VARIANT_GCS; }
+    if (gg.ok) { *eff = 0.85; return IPT_VARIANT_GCS; }
     if (slice) { *eff = 0.5; return IPT_VARIANT_SLICE_LDS; }
     return IPT_VARIANT_AUTO;
 }

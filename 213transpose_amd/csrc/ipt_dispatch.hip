@@ -64,31 +64,24 @@ static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
 
 // ---------------------------------------------------------------- GCS geometry
 
-// Geometry of the cooperative kernel.  Each member holds at most
-// KLmax tiles in LDS plus KRmax in registers: on the vector path unpadded
-// wave tiles, JL per wave in LDS and JR per wave in registers; on the scalar
-// path padded tiles, the register-held ones staged through LDS slots after
-// the first round of stores (so KRmax <= KLmax).  When every slice fits in
-// one round of one workgroup per CU, slices are spread over all CUs (larger
-// g, less data per member); otherwise members are filled to capacity.
-GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread,
-                 bool aligned) {
+// Geometry of the cooperative kernel.  Each member holds at most KLmax wave
+// tiles in LDS (JL per wave) plus KRmax in registers (JR per wave).  When
+// every slice fits in one round of one workgroup per CU, slices are spread
+// over all CUs (larger g, less data per member); otherwise members are
+// filled to capacity.  Slices of 2 GiB or more never fit (buffer offsets are
+// 32-bit, and such a slice is far beyond what 256 members hold).
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread) {
     GcsGeom gg{};
     const int64_t R = s.d2, C = s.d1, esz = s.esz;
     gg.ts = esz == 4 ? 32 : 16;
-    gg.vec = aligned && (R * esz) % 16 == 0 && (C * esz) % 16 == 0 &&
-             R * C * esz < ((int64_t)1 << 31);  // 32-bit buffer offsets
-    const int64_t elems = (int64_t)gg.ts * gg.ts;
-    const int64_t tile_bytes = (gg.vec ? elems : (int64_t)gg.ts * (gg.ts + 1)) * esz;
+    const int64_t tile_bytes = (int64_t)gg.ts * gg.ts * esz;
     const int64_t cap = std::min<int64_t>(lds_cap, lds_max_of(ctx)) - 64;  // static LDS headroom
     const int64_t nti = (R + gg.ts - 1) / gg.ts, ntj = (C + gg.ts - 1) / gg.ts;
     const int64_t ntiles = nti * ntj;
     constexpr int64_t nw = kGcsBlock / 64;
-    const int64_t klmax = gg.vec ? std::min<int64_t>(cap / tile_bytes, gcs_wave_lds_tiles(esz) * nw)
-                                 : cap / tile_bytes;
-    const int64_t krmax = gg.vec ? gcs_wave_reg_tiles(esz) * nw
-                                 : std::min<int64_t>(klmax, gcs_scalar_rv(esz) * kGcsBlock / elems);
-    gg.ok = klmax >= 1 && ntiles < ((int64_t)1 << 31) && R * C < ((int64_t)1 << 40);
+    const int64_t klmax = std::min<int64_t>(cap / tile_bytes, gcs_wave_lds_tiles(esz) * nw);
+    const int64_t krmax = gcs_wave_reg_tiles(esz) * nw;
+    gg.ok = klmax >= 1 && R * C * esz < ((int64_t)1 << 31);
     if (!gg.ok) return gg;
     const int64_t ncu = ncu_of(ctx);
     const int64_t gmin = (ntiles + klmax + krmax - 1) / (klmax + krmax);
@@ -243,7 +236,7 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
 
 // ---------------------------------------------------------------- GCS
 
-// Signature shared by both GCS kernels (k_gcs2: vector path, k_gcs: scalar).
+// Signature of the GCS kernel (both element sizes).
 template <typename T>
 using GcsKernel = void (*)(T*, uint32_t, uint32_t, uint64_t, uint32_t, uint32_t, uint32_t, Div32,
                            uint32_t, uint32_t*, uint32_t*, uint32_t);
@@ -295,11 +288,13 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
 
 template <typename T>
 static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t, hipStream_t st) {
-    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, ((uintptr_t)data % 16) == 0);
+    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true);
     if (!gg.ok) return IPT_ERR_UNSUPPORTED;
-    constexpr int TS = sizeof(T) == 4 ? 32 : 16;
-    if (gg.vec) return launch_gcs<T>(ctx, k_gcs2<T, kGcsBlock>, data, s, gg, st);
-    return launch_gcs<T>(ctx, k_gcs<T, TS, kGcsBlock, gcs_scalar_rv(sizeof(T))>, data, s, gg, st);
+    switch (gcs_jr_class(sizeof(T), gg.K, gg.KL, kGcsBlock / 64)) {
+    case 0: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 0)>, data, s, gg, st);
+    case 1: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 1)>, data, s, gg, st);
+    default: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 2)>, data, s, gg, st);
+    }
 }
 
 // ---------------------------------------------------------------- passes

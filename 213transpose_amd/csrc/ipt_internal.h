@@ -107,7 +107,6 @@ constexpr int kGcsBlock = 512;  // 8 waves per workgroup
 
 struct GcsGeom {
     int ts;           // tile side
-    bool vec;         // 16-byte global accesses
     uint32_t ntj, ntiles;
     uint32_t g;       // workgroups per slice
     uint32_t K;       // tiles per member (max)
@@ -117,12 +116,7 @@ struct GcsGeom {
 };
 
 // spread: when every slice fits one round, use more members per slice.
-// aligned: the data pointer is 16-byte aligned (the vector path needs it as
-// well as both dimensions being multiples of 16 bytes).  The scalar path
-// holds at least as many tiles per member as the vector path, so a shape
-// feasible with aligned data is feasible without.
-GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true,
-                 bool aligned = true);
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true);
 int64_t slice_pitch(int64_t C);
 int64_t lds_max_of(const DeviceCtx* ctx);
 int ncu_of(const DeviceCtx* ctx);

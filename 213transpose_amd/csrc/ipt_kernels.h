@@ -509,6 +509,7 @@ __device__ __forceinline__ void group_sync(uint32_t* bar, uint32_t* ctl, uint32_
 // code has no branches around its memory operations and every load of a
 // wave is in flight at once.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 
 constexpr uint32_t kOutside = 0x80000000u;  // past every resource built here (< 2^31 bytes)
@@ -532,8 +533,15 @@ __device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
 // rows PER*a + k (k < PER) at columns PER*b .. PER*b + PER - 1, so eight
 // lanes cover one 128-byte row segment; it transposes its micro-tile in
 // registers and stores output rows PER*b + q at columns PER*a .., eight lanes
-// again covering one 128-byte segment.  Since both dimensions are multiples
-// of PER, a micro-tile lies wholly inside or wholly outside the matrix.
+// again covering one 128-byte segment.
+// Any row length works: 16-byte buffer accesses need only 4-byte alignment
+// on gfx950, and the range check is per dword (a load past the end of the
+// slice returns zeros there, a store there is dropped).  A micro-row that
+// runs past the end of its matrix row therefore loads harmless extra values
+// (the next row's, or zeros), which land in output rows >= cols or output
+// columns >= rows.  Output rows >= cols are not stored; an output micro-row
+// that runs past `rows` (only in tiles of the last row band) is stored
+// element by element, up to `rows`.
 template <typename T>
 struct WaveTile {
     static constexpr uint32_t PER = 16 / sizeof(T);
@@ -569,13 +577,32 @@ struct WaveTile {
     // columns); `live` false stores nothing.
     __device__ __forceinline__ void store_t(Rsrc rs, uint32_t r0, uint32_t c0, uint32_t rows,
                                             uint32_t cols, uint32_t lane, bool live) const {
+        constexpr uint32_t E = sizeof(T);
         const uint32_t orow = c0 + PER * (lane & 7), ocol = r0 + PER * (lane >> 3);
-        const bool ok = live && orow < cols && ocol < rows;
-        const uint32_t o = (orow * rows + ocol) * (uint32_t)sizeof(T);
+        const bool ok = live && ocol < rows;
+        const bool full = ocol + PER <= rows;
+        const uint32_t o = (orow * rows + ocol) * E;
 #pragma unroll
         for (uint32_t q = 0; q < PER; q++)
             __builtin_amdgcn_raw_buffer_store_b128(
-                v[q], rs, ok ? o + q * rows * (uint32_t)sizeof(T) : kOutside, 0, kStoreDevice);
+                v[q], rs, ok && full && orow + q < cols ? o + q * rows * E : kOutside, 0, kStoreDevice);
+        if (r0 + TS > rows) {  // last row band (uniform across the wave)
+#pragma unroll
+            for (uint32_t q = 0; q < PER; q++) {
+                union { u32x4 v; T e[PER]; } x;
+                x.v = v[q];
+#pragma unroll
+                for (uint32_t e = 0; e < PER; e++) {
+                    const bool w = ok && !full && orow + q < cols && ocol + e < rows;
+                    const uint32_t off = w ? o + (q * rows + e) * E : kOutside;
+                    if constexpr (sizeof(T) == 4)
+                        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)x.e[e], rs, off, 0, kStoreDevice);
+                    else
+                        __builtin_amdgcn_raw_buffer_store_b64(
+                            __builtin_bit_cast(u32x2, (uint64_t)x.e[e]), rs, off, 0, kStoreDevice);
+                }
+            }
+        }
     }
 };
 
@@ -672,25 +699,37 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n, uint32_t nx
 }
 
 // Wave tiles a wave stages in LDS (JL) and holds in registers (JR) on the
-// vector path, per element size: 40 / 80 LDS tiles fill 160 KB, and the
-// register tiles hold 64 KB per 512-thread workgroup.
+// cooperative kernel, per element size: 40 / 80 LDS tiles fill 160 KB, and the
+// register tiles hold 128 KB per 512-thread workgroup.
 constexpr int gcs_wave_lds_tiles(int esz) { return esz == 4 ? 5 : 10; }
-constexpr int gcs_wave_reg_tiles(int esz) { return esz == 4 ? 2 : 4; }
+constexpr int gcs_wave_reg_tiles(int esz) { return esz == 4 ? 4 : 8; }
+// Register tiles per wave are a template argument: JR = 1, JR_max / 2 or
+// JR_max, the smallest that holds a member's tiles beyond the LDS ones
+// (unused register tiles still cost load / store instructions and
+// registers).  gcs_jr_class picks 0, 1 or 2 for those three.
+constexpr int gcs_jr_of(int esz, int cls) {
+    return cls == 0 ? 1 : cls == 1 ? gcs_wave_reg_tiles(esz) / 2 : gcs_wave_reg_tiles(esz);
+}
+inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw) {
+    const uint32_t extra = K - KL;
+    if (extra <= (uint32_t)gcs_jr_of(esz, 0) * nw) return 0;
+    if (extra <= (uint32_t)gcs_jr_of(esz, 1) * nw) return 1;
+    return 2;
+}
 
-// One vector-path GCS slot: member m of the group of g that transposes the
+// One GCS slot: member m of the group of g that transposes the
 // slice at `base`.  Member m owns tiles t = m + k*g, k < nk; wave w of the
 // workgroup takes k = w, w + NW, ...  Tiles k < kl are staged in LDS
 // (already transposed, lane-linear: conflict-free 16-byte accesses, and
 // each wave reads back only what it wrote); tiles kl <= k < nk stay in
 // registers across the group barrier.
-template <typename T, int BLOCK>
-__device__ __forceinline__ void gcs2_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
+template <typename T, int BLOCK, int JR>
+__device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
                                           uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
                                           uint32_t m, uint32_t* bar, uint32_t* ctl, u32x4* lds) {
     using W = WaveTile<T>;
     constexpr uint32_t NW = BLOCK / 64;
     constexpr int JL = gcs_wave_lds_tiles(sizeof(T));
-    constexpr int JR = gcs_wave_reg_tiles(sizeof(T));
     constexpr uint32_t TV = 64 * W::PER;  // 16-byte vectors per staged tile
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -752,11 +791,11 @@ __device__ __forceinline__ void gcs2_body(T* __restrict__ base, uint32_t R, uint
     if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
 }
 
-// Vector-path GCS kernel: slot s (numbered by xcd_slot) is member s % g of
+// GCS kernel: slot s (numbered by xcd_slot) is member s % g of
 // slice s / g.
-template <typename T, int BLOCK>
+template <typename T, int BLOCK, int JR>
 __global__ void __launch_bounds__(BLOCK)
-k_gcs2(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
+k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
        uint32_t KL, Div32 dntj, uint32_t ntiles, uint32_t* __restrict__ ctl,
        uint32_t* __restrict__ cnt, uint32_t nxcd) {
     IPT_DYN_LDS(u32x4, lds);
@@ -765,178 +804,8 @@ k_gcs2(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t 
     for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd); slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
-        gcs2_body<T, BLOCK>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
+        gcs_body<T, BLOCK, JR>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
                             g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds);
-        __syncthreads();  // LDS is reused by the next slot
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Scalar path (a dimension not a multiple of 16 bytes): TS x TS tiles of
-// single elements staged in LDS with a padded pitch TS + 1, so a column read
-// hits distinct banks; the pad column is never read.
-template <typename T, int TS>
-struct GcsTile {
-    static constexpr uint32_t TP = TS + 1;     // padded LDS pitch
-    static constexpr uint32_t TILE = TS * TP;  // elements per staged tile
-    static constexpr uint32_t VPT = TS * TS;   // elements per tile
-};
-
-// Registers each thread holds across the barrier on the scalar path.
-constexpr int gcs_scalar_rv(int esz) { return esz == 4 ? 64 : 32; }
-
-// Element loads and stores through the slice's buffer resource: 4 or 8
-// bytes, an offset of kOutside loads nothing (zero) and stores nothing.
-template <typename T>
-__device__ __forceinline__ T buf_load(Rsrc rs, uint32_t off) {
-    if constexpr (sizeof(T) == 4) {
-        return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
-    } else {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
-        return ((T)v[1] << 32) | v[0];
-    }
-}
-template <typename T>
-__device__ __forceinline__ void buf_store(Rsrc rs, uint32_t off, T x) {
-    if constexpr (sizeof(T) == 4) {
-        __builtin_amdgcn_raw_buffer_store_b32(x, rs, off, 0, kStoreDevice);
-    } else {
-        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-        const u32x2 v = {(uint32_t)x, (uint32_t)(x >> 32)};
-        __builtin_amdgcn_raw_buffer_store_b64(v, rs, off, 0, kStoreDevice);
-    }
-}
-
-// Thread geometry of the scalar path.  A tile has VPT = TS * TS elements;
-// with BLOCK >= VPT a workgroup covers S = BLOCK / VPT tiles per step, one
-// element each, else E = VPT / BLOCK elements of one tile.  The element a
-// thread moves (its row y and column x in the tile) is the same for every
-// tile, so only each tile's origin is computed per tile.
-template <int TS, int BLOCK>
-struct GcsLanes {
-    static constexpr uint32_t VPT = TS * TS;
-    static constexpr uint32_t E = VPT >= (uint32_t)BLOCK ? VPT / BLOCK : 1;
-    static constexpr uint32_t S = VPT >= (uint32_t)BLOCK ? 1 : BLOCK / VPT;
-    uint32_t ts;     // tile within a step
-    uint32_t v0;     // first element (then v0 + e * BLOCK, e < E)
-    __device__ __forceinline__ GcsLanes() {
-        ts = S > 1 ? threadIdx.x / VPT : 0;
-        v0 = S > 1 ? threadIdx.x % VPT : threadIdx.x;
-    }
-};
-
-// Writes staged tiles [k0, k0 + n) from LDS slots [0, n) transposed
-// (output row = input column), each thread the same element of every tile.
-template <typename T, int TS, int BLOCK>
-__device__ __forceinline__ void gcs_store(Rsrc rs, const T* lds, uint32_t R, uint32_t C, Div32 dntj,
-                                          uint32_t m, uint32_t g, uint32_t k0, uint32_t n) {
-    using G = GcsTile<T, TS>;
-    using L = GcsLanes<TS, BLOCK>;
-    const L ln;
-    for (uint32_t s0 = 0; s0 < n; s0 += L::S) {
-        const uint32_t k = s0 + ln.ts;
-        const uint32_t t = m + (k0 + k) * g;
-        const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
-        const bool live = k < n;
-#pragma unroll
-        for (uint32_t e = 0; e < L::E; e++) {
-            const uint32_t v = ln.v0 + e * BLOCK;
-            const uint32_t cc = v / TS, rr = v % TS;  // output row cc, column rr of the tile
-            const uint32_t orow = tj * TS + cc, ocol = ti * TS + rr;
-            const bool ok = live && orow < C && ocol < R;
-            const T x = lds[(live ? k : 0) * G::TILE + rr * G::TP + cc];
-            buf_store<T>(rs, ok ? (orow * R + ocol) * (uint32_t)sizeof(T) : kOutside, x);
-        }
-    }
-}
-
-// One scalar-path GCS slot: member m of the group of g that transposes the
-// slice at `base`.  Member m owns tiles t = m + k*g, k < nk; the first kl of
-// them are staged in LDS, the remaining nk - kl (at most RV * BLOCK / VPT)
-// stay in registers across the group barrier and go through LDS after the
-// LDS-staged tiles have been written.  Loads go through the slice's buffer
-// resource (lanes with nothing to load use kOutside), so every load of a
-// batch is in flight with no branches; a lane with nothing to stage writes
-// the pad element of slot 0, which is never read.
-template <typename T, int TS, int BLOCK, int RV>
-__device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
-                                         uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
-                                         uint32_t m, uint32_t* bar, uint32_t* ctl, T* lds) {
-    using G = GcsTile<T, TS>;
-    using L = GcsLanes<TS, BLOCK>;
-    constexpr uint32_t U = 32;          // LDS-bound elements in flight per thread
-    constexpr uint32_t PAD = TS;        // slot 0, row 0, pad column
-    constexpr uint32_t RT = RV / L::E * L::S;  // register tiles (at most)
-    const L ln;
-    const uint32_t nk = m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0;
-    const uint32_t kl = min(nk, KL), kr = nk - kl;
-    const Rsrc rs = make_rsrc(base, R * C * (uint32_t)sizeof(T));
-    // input offset (bytes) of this thread's element e of member tile k
-    auto src = [&](uint32_t k, uint32_t e, bool live) -> uint32_t {
-        const uint32_t t = m + k * g;
-        const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
-        const uint32_t v = ln.v0 + e * BLOCK;
-        const uint32_t r = ti * TS + v / TS, c = tj * TS + v % TS;
-        return live && r < R && c < C ? (r * C + c) * (uint32_t)sizeof(T) : kOutside;
-    };
-    auto slot = [&](uint32_t k, uint32_t e) -> uint32_t {
-        const uint32_t v = ln.v0 + e * BLOCK;
-        return k * G::TILE + (v / TS) * G::TP + v % TS;
-    };
-
-    // register-held tiles kl .. nk - 1: RV elements per thread, issued first
-    T rreg[RV];
-    if (kr) {
-#pragma unroll
-        for (uint32_t u = 0; u < RV; u++) {
-            const uint32_t k = (u / L::E) * L::S + ln.ts, e = u % L::E;
-            rreg[u] = buf_load<T>(rs, src(kl + k, e, k < kr));
-        }
-    }
-    // LDS-staged tiles 0 .. kl - 1, U elements per thread per batch
-    for (uint32_t k0 = 0; k0 < kl; k0 += U / L::E * L::S) {
-        T reg[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            const uint32_t k = k0 + (u / L::E) * L::S + ln.ts, e = u % L::E;
-            reg[u] = buf_load<T>(rs, src(k, e, k < kl));
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            const uint32_t k = k0 + (u / L::E) * L::S + ln.ts, e = u % L::E;
-            lds[k < kl ? slot(k, e) : PAD] = reg[u];
-        }
-    }
-    group_sync(bar, ctl, m, g);
-    gcs_store<T, TS, BLOCK>(rs, lds, R, C, dntj, m, g, 0, kl);
-    if (kr) {
-        __syncthreads();
-#pragma unroll
-        for (uint32_t u = 0; u < RV; u++) {
-            const uint32_t k = (u / L::E) * L::S + ln.ts, e = u % L::E;
-            lds[k < kr ? slot(k, e) : PAD] = rreg[u];
-        }
-        __syncthreads();
-        gcs_store<T, TS, BLOCK>(rs, lds, R, C, dntj, m, g, kl, kr);
-    }
-    if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
-    (void)RT;
-}
-
-// Scalar-path GCS kernel: slot s is member s % g of slice s / g.
-template <typename T, int TS, int BLOCK, int RV>
-__global__ void __launch_bounds__(BLOCK)
-k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
-      uint32_t KL, Div32 dntj, uint32_t ntiles, uint32_t* __restrict__ ctl,
-      uint32_t* __restrict__ cnt, uint32_t nxcd) {
-    IPT_DYN_LDS(T, lds);
-    const uint64_t total = nslices * g;
-    const uint64_t slice_elems = (uint64_t)R * C;
-    for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd); slot < total; slot += gridDim.x) {
-        const uint64_t S = slot / g;
-        const uint32_t m = (uint32_t)(slot - S * g);
-        gcs_body<T, TS, BLOCK, RV>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
-                                   g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds);
         __syncthreads();  // LDS is reused by the next slot
     }
 }
@@ -950,8 +819,7 @@ namespace ipt {
 // items.  Each instance (d3 slices of R x C, 4- or 8-byte elements) is cut
 // into items of one kind:
 //   BK_SLICE  `sub` consecutive slices staged whole in LDS (slice_body)
-//   BK_GCS    member `sub` of the group of g that owns one slice (gcs2_body
-//             on the vector path, gcs_body on the scalar path);
+//   BK_GCS    member `sub` of the group of g that owns one slice (gcs_body);
 //             the g members of a slice occupy consecutive items
 //   BK_SWAP   tile pair `sub` of one square slice (swap_body)
 // Workgroups take items from a queue in increasing order, one at a time, and
@@ -961,15 +829,15 @@ namespace ipt {
 // the earliest unfinished group completes; the frontier group gets its
 // remaining members from workgroups freed as earlier groups complete (all
 // grid workgroups would suffice).  Only group members ever wait.
-enum BatchKind : uint32_t { BK_SLICE = 0, BK_GCS_VEC = 1, BK_GCS_SCALAR = 2, BK_SWAP = 3,
-                            BK_SWAP_VEC = 4 };
+enum BatchKind : uint32_t { BK_SLICE = 0, BK_GCS = 1, BK_SWAP = 2, BK_SWAP_VEC = 3 };
 
 struct BatchInst {
     uint64_t data;        // device address of slice 0
-    uint64_t bar;         // word offset of slice 0's barrier state (BK_GCS_*)
+    uint64_t bar;         // word offset of slice 0's barrier state (BK_GCS)
     uint32_t R, C, esz, kind;
-    uint32_t g, K, KL, ntiles;   // BK_GCS_*
-    Div32 dntj;                  // BK_GCS_*
+    uint32_t g, K, KL, ntiles;   // BK_GCS
+    Div32 dntj;                  // BK_GCS
+    uint32_t jr;                 // BK_GCS: register-tile class (gcs_jr_class)
     uint32_t P;                  // BK_SLICE: LDS row pitch
     Div32 dS, dC, dR;            // BK_SLICE
 };
@@ -985,23 +853,26 @@ constexpr int kBatchBlock = 512;
 template <typename T>
 __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& it, uint32_t* ctl,
                                            uint32_t* bars, T* lds) {
-    constexpr int TS_GCS = sizeof(T) == 4 ? 32 : 16;
     constexpr int TS_SWAP = sizeof(T) == 4 ? 64 : 32;
     T* base = reinterpret_cast<T*>(I.data) + it.slice * (uint64_t)I.R * I.C;
     switch (I.kind) {
     case BK_SLICE:
         slice_body<T>(base, I.R, I.C, I.P, it.sub, I.dS, I.dC, I.dR, lds);
         break;
-    case BK_GCS_VEC:
-        gcs2_body<T, kBatchBlock>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles, it.sub,
-                                  bars + I.bar + bar_words(I.g) * it.slice, ctl,
-                                  reinterpret_cast<u32x4*>(lds));
+    case BK_GCS: {
+        uint32_t* bar = bars + I.bar + bar_words(I.g) * it.slice;
+        u32x4* l = reinterpret_cast<u32x4*>(lds);
+        if (I.jr == 0)
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0)>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+                                                              I.ntiles, it.sub, bar, ctl, l);
+        else if (I.jr == 1)
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1)>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+                                                              I.ntiles, it.sub, bar, ctl, l);
+        else
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 2)>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+                                                              I.ntiles, it.sub, bar, ctl, l);
         break;
-    case BK_GCS_SCALAR:
-        gcs_body<T, TS_GCS, kBatchBlock, gcs_scalar_rv(sizeof(T))>(
-            base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles, it.sub,
-            bars + I.bar + bar_words(I.g) * it.slice, ctl, lds);
-        break;
+    }
     case BK_SWAP_VEC:
         swap_body<T, TS_SWAP, kBatchBlock, true>(base, I.R, it.sub, lds);
         break;

@@ -100,7 +100,6 @@ def kernels_of(ipt, d1, d2, d3, esz, tuned=None):
               ipt.VARIANT_GCS: "k_gcs"}
     if v in single:
         fn = (lambda t: ipt.transpose_variant(t, d1, d2, d3, v, tune=tune))
-        # k_gcs2<T, ...> (vector path) and k_gcs<T, ...> (scalar path) both match
         return v, [(ipt.VARIANT_NAMES[v], fn, (single[v], f"<{tname}"))]
     # C2R views slices as R = d2 rows x C = d1 columns, R2C as R = d1, C = d2
     R, C = (d2, d1) if v == ipt.VARIANT_C2R else (d1, d2)

@@ -114,7 +114,7 @@ def test_block_cycle_unaligned_data_falls_back(ipt, oracle):
 @pytest.mark.parametrize("esz", [4, 8])
 def test_gcs_groups_and_barriers(ipt, oracle, esz):
     """Cooperative kernel with large groups (small LDS caps), many slices per
-    launch (several rounds of slots), unaligned shapes (scalar path) and
+    launch (several rounds of slots), rows of any length and
     back-to-back launches (counters restored to zero by each launch)."""
     cases = [((512, 384, 64), None), ((256, 1000, 17), None), ((3000, 1000, 1), None),
              ((97, 203, 40), ipt.Tune(lds_bytes=8192)), ((300, 100, 9), ipt.Tune(lds_bytes=4608)),
@@ -134,6 +134,34 @@ def test_gcs_groups_and_barriers(ipt, oracle, esz):
             ran += 1
         ipt.device_check()
     assert ran >= 8
+
+
+@pytest.mark.parametrize("esz", [4, 8])
+def test_gcs_any_row_length_and_base(ipt, oracle, esz):
+    """The cooperative kernel's 16-byte tiles on rows of any length and on a
+    base one element past a 16-byte boundary: partial micro-rows at the right
+    and bottom edges, loads that run past the end of the slice, element-wise
+    stores in the last row band."""
+    cases = [(1001, 999, 3), (333, 77, 40), (97, 203, 5), (4093, 4099, 1), (35, 33, 300),
+             (1023, 1, 1), (1, 1023, 2), (17, 4099, 2), (4099, 17, 2)]
+    ran = 0
+    for k, (d1, d2, d3) in enumerate(cases):
+        if not ipt.feasible(ipt.VARIANT_GCS, esz, d1, d2, d3):
+            continue
+        for off in (0, 1):
+            print(f"[case] gcs d1={d1} d2={d2} d3={d3} esz={esz} offset={off}", flush=True)
+            n = d1 * d2 * d3
+            a = oracle.random_bits(n + 1, esz, 700 + 10 * k + off)
+            t = to_dev(a)
+            ipt.transpose_variant(t[off:off + n], d1, d2, d3, ipt.VARIANT_GCS)
+            torch.cuda.synchronize()
+            got = to_host(t, esz)
+            exp = a.copy()
+            exp[off:off + n] = oracle.seq_tt_213(a[off:off + n], d1, d2, d3)
+            assert np.array_equal(got, exp), (d1, d2, d3, esz, off)
+            ran += 1
+    ipt.device_check()
+    assert ran >= 12
 
 
 @pytest.mark.parametrize("esz", [4, 8])
