@@ -33,7 +33,8 @@ ipt = importlib.import_module("213transpose_amd")
 # The sweep's domain is truncated at MAX_BYTES per shape: timing every
 # candidate on the multi-GB tail of the log-uniform distribution would cost
 # GPU-hours.  The selector decides only inside this domain (see
-# SELECTOR_MAX_BYTES in the library); larger shapes keep the heuristic.
+# kSelectorMaxBytes in the library); larger shapes, and shapes with a side
+# beyond the sweep's 65536 (kSelectorMaxSide), keep the heuristic.
 MAX_BYTES = 1 << 30
 
 

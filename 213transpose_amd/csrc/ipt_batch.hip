@@ -153,7 +153,7 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
 int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t lds_cap, int grid) {
     const int tuned = tuned_candidate(s);
     if (tuned >= 0) return tuned;
-    if (!y || (uint64_t)s.d1 * s.d2 * s.d3 * s.esz > kSelectorMaxBytes) return -1;
+    if (!y || !selector_domain(s.d1, s.d2, s.d3, s.esz)) return -1;
     const int64_t slice_bytes = s.d2 * slice_pitch(s.d1) * s.esz;
     const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false);
     const bool route_ok[4] = {slice_bytes <= lds_cap, s.d1 == s.d2,

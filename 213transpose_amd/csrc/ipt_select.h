@@ -17,9 +17,15 @@ struct Candidate {
 constexpr int kNumCandidates = IPT_NUM_CANDIDATES;
 extern const Candidate kCandidates[kNumCandidates];
 
-// The selector decides for problems up to this many payload bytes (the
-// domain its timing sweep covers); larger ones keep the heuristic.
+// The selector decides only inside the domain its timing sweep covers:
+// payloads up to kSelectorMaxBytes and both sides up to kSelectorMaxSide;
+// other shapes keep the heuristic (the network does not extrapolate).
 constexpr uint64_t kSelectorMaxBytes = (uint64_t)1 << 30;
+constexpr int64_t kSelectorMaxSide = 65536;
+inline bool selector_domain(int64_t d1, int64_t d2, int64_t d3, int esz) {
+    return d1 <= kSelectorMaxSide && d2 <= kSelectorMaxSide &&
+           (uint64_t)d1 * d2 * d3 * esz <= kSelectorMaxBytes;
+}
 
 // Candidate the plain heuristic picks for a shape (maps choose_variant onto
 // the candidate list).

@@ -142,7 +142,7 @@ static void load_default_weights_locked() {
 }
 
 int select_candidate(const DeviceCtx* ctx, const Shape& s) {
-    const bool in_domain = (uint64_t)s.d1 * s.d2 * s.d3 * s.esz <= kSelectorMaxBytes;
+    const bool in_domain = selector_domain(s.d1, s.d2, s.d3, s.esz);
     {
         std::lock_guard<std::mutex> g(g_sel_mu);
         auto tuned = g_tuned.find(std::make_tuple(s.esz, s.d1, s.d2, s.d3));

@@ -72,7 +72,69 @@ def parse():
                     help="build the uneven batch and run its graph a few times (for a profiler)")
     ap.add_argument("--profile", action="store_true",
                     help="only run workload steps and the calibration copies (for profilers)")
+    ap.add_argument("--configs", action="store_true",
+                    help="time every reference config shape with the default dispatch (one JSON line)")
+    ap.add_argument("--only", default=None,
+                    help="--configs: comma-separated config names to run (default: all)")
     return ap.parse_args()
+
+
+CONFIGS = [  # (name, (d1, d2, d3, esz)): the reference's config shapes
+    ("1", (3000, 1000, 1, 4)), ("2a", (512, 384, 64, 4)), ("2b", (256, 1000, 17, 8)),
+    ("3a", (16777216, 32, 1, 8)), ("3b", (7, 4194304, 1, 4)), ("3c", (4099, 4093, 1, 4)),
+    ("3d", (16384, 16384, 1, 4)),
+]
+
+
+def configs_main(args, torch, ipt):
+    """Every config shape, iota input: one transpose checked against the
+    gen_ans digest and undone; then, for the transpose and for its inverse
+    apart, 3 warm-up calls and the median over 5 event-timed blocks of 10
+    back-to-back calls.  The work of an in-place permutation does not depend
+    on the values, so repeated calls time the same work."""
+    import numpy as np
+    out = []
+    only = set(args.only.split(",")) if args.only else None
+    for name, (d1, d2, d3, esz) in CONFIGS:
+        if only and name not in only:
+            continue
+        dt = torch.int32 if esz == 4 else torch.int64
+        t = torch.arange(d1 * d2 * d3, dtype=dt, device="cuda")
+        ipt.transpose(t, d1, d2, d3)
+        torch.cuda.synchronize()
+        sha = hashlib.sha256(t.cpu().numpy().tobytes()).hexdigest()
+        ok = sha == golden_sha((d1, d2, d3, esz))
+        ipt.transpose(t, d2, d1, d3)
+        def per_call(a, b):
+            for _ in range(3):
+                ipt.transpose(t, a, b, d3)
+            blocks = []
+            for _ in range(5):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(10):
+                    ipt.transpose(t, a, b, d3)
+                e1.record()
+                e1.synchronize()
+                blocks.append(e0.elapsed_time(e1) / 10)
+            return float(np.median(blocks))
+        ms = per_call(d1, d2)
+        ms_inv = per_call(d2, d1)
+        ipt.device_check()
+        nb = 2 * d1 * d2 * d3 * esz
+        k = ipt.select_candidate(esz, d1, d2, d3)
+        out.append({"config": name, "shape": [d1, d2, d3, esz], "verified_vs_gen_ans": ok,
+                    "variant": ipt.VARIANT_NAMES[ipt.candidate(k)[0]], "ms": round(ms, 4),
+                    "GBps": round(nb / (ms * 1e-3) / 1e9, 1),
+                    "frac_of_hbm_peak": round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "inverse_variant": ipt.VARIANT_NAMES[ipt.candidate(
+                        ipt.select_candidate(esz, d2, d1, d3))[0]],
+                    "inverse_GBps": round(nb / (ms_inv * 1e-3) / 1e9, 1)})
+        print(f"[config {name}] {out[-1]}", file=sys.stderr, flush=True)
+        del t
+        torch.cuda.empty_cache()
+    print(json.dumps({"configs": out, "timing": "median of 5 HIP-event-timed blocks of 10 "
+                                                "back-to-back calls, default dispatch"}))
 
 
 def golden_sha(shape):
@@ -327,6 +389,9 @@ def main():
         return
     if args.profile_batch:
         profile_batch_main(args, torch, ipt)
+        return
+    if args.configs:
+        configs_main(args, torch, ipt)
         return
     tensors = make_tensors(torch)
 
