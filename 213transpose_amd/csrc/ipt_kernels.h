@@ -574,7 +574,8 @@ struct WaveTile {
     }
     // After transpose(): the tile loaded at (r0, c0) of a rows x cols slice
     // goes to output rows c0 + .., columns r0 + .. (the output has `rows`
-    // columns); `live` false stores nothing.
+    // columns); `live` false stores nothing.  SP: the stores' cache policy.
+    template <int SP>
     __device__ __forceinline__ void store_t(Rsrc rs, uint32_t r0, uint32_t c0, uint32_t rows,
                                             uint32_t cols, uint32_t lane, bool live) const {
         constexpr uint32_t E = sizeof(T);
@@ -585,7 +586,7 @@ struct WaveTile {
 #pragma unroll
         for (uint32_t q = 0; q < PER; q++)
             __builtin_amdgcn_raw_buffer_store_b128(
-                v[q], rs, ok && full && orow + q < cols ? o + q * rows * E : kOutside, 0, kStoreDevice);
+                v[q], rs, ok && full && orow + q < cols ? o + q * rows * E : kOutside, 0, SP);
         if (r0 + TS > rows) {  // last row band (uniform across the wave)
 #pragma unroll
             for (uint32_t q = 0; q < PER; q++) {
@@ -596,10 +597,10 @@ struct WaveTile {
                     const bool w = ok && !full && orow + q < cols && ocol + e < rows;
                     const uint32_t off = w ? o + (q * rows + e) * E : kOutside;
                     if constexpr (sizeof(T) == 4)
-                        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)x.e[e], rs, off, 0, kStoreDevice);
+                        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)x.e[e], rs, off, 0, SP);
                     else
                         __builtin_amdgcn_raw_buffer_store_b64(
-                            __builtin_bit_cast(u32x2, (uint64_t)x.e[e]), rs, off, 0, kStoreDevice);
+                            __builtin_bit_cast(u32x2, (uint64_t)x.e[e]), rs, off, 0, SP);
                 }
             }
         }
@@ -723,7 +724,7 @@ inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw) {
 // (already transposed, lane-linear: conflict-free 16-byte accesses, and
 // each wave reads back only what it wrote); tiles kl <= k < nk stay in
 // registers across the group barrier.
-template <typename T, int BLOCK, int JR>
+template <typename T, int BLOCK, int JR, int SP>
 __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
                                           uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
                                           uint32_t m, uint32_t* bar, uint32_t* ctl, u32x4* lds) {
@@ -778,7 +779,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
             for (uint32_t q = 0; q < W::PER; q++) x.v[q] = lds[k * TV + q * 64 + lane];
             uint32_t r0, c0;
             origin(k, &r0, &c0);
-            x.store_t(rs, r0, c0, R, C, lane, true);
+            x.template store_t<SP>(rs, r0, c0, R, C, lane, true);
         }
     }
 #pragma unroll
@@ -786,7 +787,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
         const uint32_t k = kl + wave + j * NW;
         uint32_t r0, c0;
         origin(k, &r0, &c0);
-        rt[j].store_t(rs, r0, c0, R, C, lane, k < nk);
+        rt[j].template store_t<SP>(rs, r0, c0, R, C, lane, k < nk);
     }
     if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
 }
@@ -804,7 +805,7 @@ k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g
     for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd); slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
-        gcs_body<T, BLOCK, JR>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
+        gcs_body<T, BLOCK, JR, kStoreDevice>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
                             g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds);
         __syncthreads();  // LDS is reused by the next slot
     }
@@ -860,16 +861,21 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
         slice_body<T>(base, I.R, I.C, I.P, it.sub, I.dS, I.dC, I.dR, lds);
         break;
     case BK_GCS: {
+        // Plain (write-back) stores: a long-running persistent kernel does not
+        // pay for the end-of-kernel write-back that device-scope stores avoid
+        // for single launches, and L2 merges the partial lines of unaligned
+        // rows (uneven 4096-matrix batch, its cooperative groups: 3.74 ->
+        // 3.33 ms).
         uint32_t* bar = bars + I.bar + bar_words(I.g) * it.slice;
         u32x4* l = reinterpret_cast<u32x4*>(lds);
         if (I.jr == 0)
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0)>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
                                                               I.ntiles, it.sub, bar, ctl, l);
         else if (I.jr == 1)
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1)>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
                                                               I.ntiles, it.sub, bar, ctl, l);
         else
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 2)>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 2), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
                                                               I.ntiles, it.sub, bar, ctl, l);
         break;
     }

@@ -122,6 +122,27 @@ def test_batch_block_cycle_instances(ipt, oracle):
 
 
 @pytest.mark.gpu
+def test_tiny_batches(ipt, oracle):
+    """Batches with fewer work items than twice the XCD count run one shared
+    queue (and a grid of a few workgroups)."""
+    for k, shapes in enumerate([[(300, 200, 1, 4)], [(300, 200, 1, 8), (5, 7, 1, 4)],
+                                [(64, 64, 1, 4)], [(200, 300, 2, 4)]]):
+        buf, items, host = make_instances(oracle, shapes, 1300 + 10 * k)
+        b = ipt.Batch(items)
+        info = b.info()
+        print("[tiny batch]", shapes, info, flush=True)
+        b.run(graph=False)
+        torch.cuda.synchronize()
+        check(oracle, items, host, 1)
+        b.run(graph=True)
+        b.run(graph=True)
+        torch.cuda.synchronize()
+        b.check()
+        check(oracle, items, host, 3)
+        b.close()
+
+
+@pytest.mark.gpu
 def test_uneven_workload_sample(ipt, oracle):
     """A 300-instance sample of the uneven-batch workload (m, n log-uniform in
     [2, 1024]), one graph launch, every instance checked."""
