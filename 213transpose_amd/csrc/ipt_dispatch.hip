@@ -69,8 +69,15 @@ static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
 // every slice fits in one round of one workgroup per CU, slices are spread
 // over all CUs (larger g, less data per member); otherwise members are
 // filled to capacity.  Slices of 2 GiB or more never fit (buffer offsets are
-// 32-bit, and such a slice is far beyond what 256 members hold).
-GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread) {
+// 32-bit, and such a slice is far beyond what 256 members hold).  The largest
+// register-tile class is used only when a slice needs it to fit: members
+// that large were slower wherever a smaller class also fits (f32 3000 x 3000
+// x 8: 3446 vs 4578 GB/s).
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread, int jr_cls) {
+    if (jr_cls > 1) {
+        const GcsGeom small = gcs_geom(ctx, s, lds_cap, spread, 1);
+        if (small.ok) return small;
+    }
     GcsGeom gg{};
     const int64_t R = s.d2, C = s.d1, esz = s.esz;
     gg.ts = esz == 4 ? 32 : 16;
@@ -80,7 +87,7 @@ GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spr
     const int64_t ntiles = nti * ntj;
     constexpr int64_t nw = kGcsBlock / 64;
     const int64_t klmax = std::min<int64_t>(cap / tile_bytes, gcs_wave_lds_tiles(esz) * nw);
-    const int64_t krmax = gcs_wave_reg_tiles(esz) * nw;
+    const int64_t krmax = (int64_t)gcs_jr_of(esz, jr_cls) * nw;
     gg.ok = klmax >= 1 && R * C * esz < ((int64_t)1 << 31);
     if (!gg.ok) return gg;
     const int64_t ncu = ncu_of(ctx);

@@ -116,7 +116,10 @@ struct GcsGeom {
 };
 
 // spread: when every slice fits one round, use more members per slice.
-GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true);
+// jr_cls: the largest register-tile class the kernel that runs it has
+// (gcs_jr_of; the batch kernel stops at kBatchJrClass).
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true,
+                 int jr_cls = 2);
 int64_t slice_pitch(int64_t C);
 int64_t lds_max_of(const DeviceCtx* ctx);
 int ncu_of(const DeviceCtx* ctx);

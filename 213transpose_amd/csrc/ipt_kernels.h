@@ -703,11 +703,14 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n, uint32_t nx
 // cooperative kernel, per element size: 40 / 80 LDS tiles fill 160 KB, and the
 // register tiles hold 128 KB per 512-thread workgroup.
 constexpr int gcs_wave_lds_tiles(int esz) { return esz == 4 ? 5 : 10; }
-constexpr int gcs_wave_reg_tiles(int esz) { return esz == 4 ? 4 : 8; }
+constexpr int gcs_wave_reg_tiles(int esz) { return esz == 4 ? 8 : 16; }
+constexpr int kBatchJrClass = 1;
 // Register tiles per wave are a template argument: JR = 1, JR_max / 2 or
 // JR_max, the smallest that holds a member's tiles beyond the LDS ones
 // (unused register tiles still cost load / store instructions and
-// registers).  gcs_jr_class picks 0, 1 or 2 for those three.
+// registers).  gcs_jr_class picks 0, 1 or 2 for those three.  JR_max
+// (235 / 228 VGPRs) fits the two waves per SIMD that LDS allows; the batch
+// kernel, which also holds the other item kinds, stops at class 1.
 constexpr int gcs_jr_of(int esz, int cls) {
     return cls == 0 ? 1 : cls == 1 ? gcs_wave_reg_tiles(esz) / 2 : gcs_wave_reg_tiles(esz);
 }
@@ -868,14 +871,12 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
         // 3.33 ms).
         uint32_t* bar = bars + I.bar + bar_words(I.g) * it.slice;
         u32x4* l = reinterpret_cast<u32x4*>(lds);
+        static_assert(kBatchJrClass == 1, "one body per register-tile class");
         if (I.jr == 0)
             gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
                                                               I.ntiles, it.sub, bar, ctl, l);
-        else if (I.jr == 1)
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
-                                                              I.ntiles, it.sub, bar, ctl, l);
         else
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 2), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
                                                               I.ntiles, it.sub, bar, ctl, l);
         break;
     }
