@@ -323,8 +323,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     size_t head_bytes = 0;
     auto add_block_cycle = [&](void* data, const Shape& s, ipt_status* err) -> bool {
         *err = IPT_OK;
-        if ((uintptr_t)data % 16 != 0) return false;
-        const BlockCyclePlan bp = block_cycle_plan(ctx, s);
+        const BlockCyclePlan bp = block_cycle_for(block_cycle_plan(ctx, s), data);
         if (!bp.ok) return false;
         int bv = IPT_VARIANT_AUTO;
         const Route rb = plan_instance(ctx, bp.blk, bp.block_variant, lds_cap, grid, b->tune, &bv);
@@ -346,7 +345,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         head_bytes = std::max<size_t>(head_bytes, std::min<uint64_t>(group, s.d3) * per_slice);
         *err = add_items(data, bp.blk, rb);
         if (*err != IPT_OK) return false;
-        const ipt_batch::Chunk c{data, (uint64_t)s.d3, (uint64_t)s.d1 * s.d2 * s.esz / 16, group,
+        const ipt_batch::Chunk c{data, (uint64_t)s.d3, (uint64_t)s.d1 * s.d2 * s.esz / (uint64_t)bp.vb, group,
                                  bp, it->second};
         (bp.chunk_first ? b->pre : b->post).push_back(c);
         b->info.n_block_cycle++;

@@ -17,7 +17,8 @@
 // segments on the host once per chunk-matrix shape (cached per device);
 // every segment's first chunk is saved to scratch, and then each segment is
 // moved by one workgroup with no synchronisation between segments.  Both
-// steps stream contiguous 16-byte vectors: about two sweeps.
+// steps stream contiguous vectors (16 bytes where the chunk size and the
+// data's alignment allow, else 8 or 4): about two sweeps.
 #include "ipt_internal.h"
 #include "ipt_kernels.h"
 
@@ -32,7 +33,7 @@ namespace ipt {
 
 // ---------------------------------------------------------------------------
 // Chunk permutation.  A slice is a sequence of N = rc * cc chunks of cv
-// 16-byte vectors, and the content of chunk q moves to
+// vectors (V: 16, 8 or 4 bytes), and the content of chunk q moves to
 //     P(q) = (q mod cc) * rc + q / cc
 // (the transpose of an rc x cc matrix of chunks).  The host walks the cycles
 // of P once per shape and writes a move plan: work item i is moves
@@ -54,15 +55,16 @@ constexpr uint32_t kHeadFlag = 0x80000000u;
 
 // head[(slice * nheads + h) * cv + v] = vector v of chunk hpos[h] of the
 // slice.  Lanes past the end repeat the last vector (same address, value).
+template <typename V>
 __global__ void __launch_bounds__(kChunkBlock)
-k_chunk_save(const uint4* __restrict__ data, uint4* __restrict__ head, const uint32_t* __restrict__ hpos,
+k_chunk_save(const V* __restrict__ data, V* __restrict__ head, const uint32_t* __restrict__ hpos,
              uint32_t nheads, uint64_t nslices, uint64_t slice_vecs, uint32_t cv, Div32 dcv) {
     const uint64_t per_slice = (uint64_t)nheads * cv;
     const uint64_t total = nslices * per_slice;
     constexpr uint32_t U = 4;
     for (uint64_t e0 = (uint64_t)blockIdx.x * kChunkBlock * U; e0 < total;
          e0 += (uint64_t)gridDim.x * kChunkBlock * U) {
-        uint4 r[U];
+        V r[U];
         uint64_t dst[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
@@ -83,11 +85,12 @@ k_chunk_save(const uint4* __restrict__ data, uint4* __restrict__ head, const uin
 // slice), then each staged chunk written to its destination.  Staging lanes
 // past the end repeat the last vector, so the loads carry no branches and a
 // batch of U of them is in flight at once.
+template <typename V>
 __global__ void __launch_bounds__(kChunkBlock)
-k_chunk_cycle(uint4* __restrict__ data, const uint4* __restrict__ head, const uint2* __restrict__ items,
+k_chunk_cycle(V* __restrict__ data, const V* __restrict__ head, const uint2* __restrict__ items,
               const uint32_t* __restrict__ msrc, const uint32_t* __restrict__ mdst, uint32_t nitems,
               uint32_t nheads, uint64_t nslices, uint64_t slice_vecs, uint32_t cv, Div32 dcv) {
-    extern __shared__ __align__(16) uint4 stage[];
+    IPT_DYN_LDS(V, stage);
     __shared__ uint32_t src[kBcMaxSeg], dst[kBcMaxSeg];
     constexpr uint32_t U = 8;
     const uint64_t total_items = nslices * nitems;
@@ -95,7 +98,7 @@ k_chunk_cycle(uint4* __restrict__ data, const uint4* __restrict__ head, const ui
         const uint64_t sl = it / nitems;
         const uint2 item = items[it - sl * nitems];
         const uint32_t nm = item.y;
-        uint4* base = data + sl * slice_vecs;
+        V* base = data + sl * slice_vecs;
         for (uint32_t j = threadIdx.x; j < nm; j += kChunkBlock) {
             src[j] = msrc[item.x + j];
             dst[j] = mdst[item.x + j];
@@ -103,13 +106,13 @@ k_chunk_cycle(uint4* __restrict__ data, const uint4* __restrict__ head, const ui
         __syncthreads();
         uint32_t first = 0;
         if (src[0] & kHeadFlag) {  // a cut cycle's segment: its head was saved
-            const uint4* h = head + (sl * nheads + (src[0] & ~kHeadFlag)) * cv;
+            const V* h = head + (sl * nheads + (src[0] & ~kHeadFlag)) * cv;
             for (uint32_t v = threadIdx.x; v < cv; v += kChunkBlock) stage[v] = h[v];
             first = 1;
         }
         const uint32_t total = (nm - first) * cv;
         for (uint32_t e0 = 0; e0 < total; e0 += U * kChunkBlock) {
-            uint4 r[U];
+            V r[U];
             uint32_t at[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
@@ -134,11 +137,13 @@ k_chunk_cycle(uint4* __restrict__ data, const uint4* __restrict__ head, const ui
 // ---------------------------------------------------------------------------
 // Plan.  Candidates: both orientations, every divisor T of the chunked side
 // with T <= L / 2 (at least two blocks), a chunk of kBcMinChunk ..
-// kBcMaxChunk bytes (a multiple of 16), fewer than kBcMaxChunks per slice and
-// a block some one-sweep kernel runs.  Each is priced in sweeps:
+// kBcMaxChunk bytes, fewer than kBcMaxChunks per slice and a block some
+// one-sweep kernel runs.  Chunks move as 16-byte vectors when their size is
+// a multiple of 16, else as 8- or 4-byte ones.  Each is priced in sweeps:
 //   chunk step  (1 + 1.5 / ell): the saved heads are read and written once
 //               and read again, 1/ell of the data; chunks under 2 KB cost
-//               extra (short contiguous runs per position);
+//               extra (short contiguous runs per position), and so do
+//               narrower vectors (x1.15 for 8 bytes, x1.4 for 4);
 //   block step  1 / efficiency of its kernel relative to a copy (measured:
 //               square swap ~1, whole-slice 16-byte form ~0.9, cooperative
 //               kernel ~0.85, whole-slice scalar form ~0.5).
@@ -161,7 +166,7 @@ static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, Bl
     const int64_t tmax = std::min<int64_t>(L / 2, kBcMaxChunk / esz);
     const int64_t tmin = (kBcMinChunk + esz - 1) / esz;
     for (int64_t T = tmax; T >= tmin; T--) {
-        if (L % T || (T * esz) % 16) continue;
+        if (L % T) continue;
         const int64_t nb = L / T;
         if (nb * other >= kBcMaxChunks) break;  // smaller T only adds chunks
         const Shape blk = cols ? make_shape(s.esz, T, other, s.d3 * nb)
@@ -170,9 +175,11 @@ static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, Bl
         const int v = block_kernel(ctx, blk, &eff);
         if (v == IPT_VARIANT_AUTO) continue;
         const int64_t chunk = T * esz;
+        const int64_t vb = chunk % 16 == 0 ? 16 : chunk % 8 == 0 ? 8 : 4;
         const int64_t ell = std::max<int64_t>(1, std::min<int64_t>(kBcMaxSeg, kBcStageBytes / chunk));
         const double short_run = chunk < 2048 ? 1.0 + (2048.0 - chunk) / 4096.0 : 1.0;
-        const double cost = (1.0 + 1.5 / ell) * short_run + 1.0 / eff;
+        const double narrow = vb == 16 ? 1.0 : vb == 8 ? 1.15 : 1.4;
+        const double cost = (1.0 + 1.5 / ell) * short_run * narrow + 1.0 / eff;
         if (best->ok && cost >= *best_cost - 1e-9) continue;  // ties keep the larger chunk
         best->ok = true;
         best->chunk_first = cols;
@@ -181,12 +188,25 @@ static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, Bl
         best->other = other;
         best->rc = cols ? other : nb;
         best->cc = cols ? nb : other;
-        best->cv = chunk / 16;
+        best->vb = vb;
+        best->cv = chunk / vb;
         best->ell = ell;
         best->blk = blk;
         best->block_variant = v;
         *best_cost = cost;
     }
+}
+
+BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data) {
+    BlockCyclePlan q = p;
+    if (!q.ok) return q;
+    while (q.vb > 4 && (uintptr_t)data % (uintptr_t)q.vb != 0) q.vb /= 2;
+    if ((uintptr_t)data % 4 != 0) {
+        q.ok = false;
+        return q;
+    }
+    q.cv = q.T * q.blk.esz / q.vb;
+    return q;
 }
 
 BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s) {
@@ -332,10 +352,40 @@ ipt_status chunk_kernels_setup(const DeviceCtx* ctx) {
     static std::set<int> raised;
     std::lock_guard<std::mutex> g(g_plan_mu);
     if (!raised.count(ctx->dev)) {
-        IPT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chunk_cycle),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)kBcStageBytes));
+        for (const void* k : {reinterpret_cast<const void*>(k_chunk_cycle<uint4>),
+                              reinterpret_cast<const void*>(k_chunk_cycle<uint2>),
+                              reinterpret_cast<const void*>(k_chunk_cycle<uint32_t>)})
+            IPT_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)kBcStageBytes));
         raised.insert(ctx->dev);
+    }
+    return IPT_OK;
+}
+
+template <typename V>
+static ipt_status chunk_step_launch_v(const DeviceCtx* ctx, void* data, uint64_t nslices,
+                                     uint64_t slice_vecs, const BlockCyclePlan& p,
+                                     const ChunkMoves& m, void* head, uint64_t group,
+                                     hipStream_t st) {
+    const size_t lds = (size_t)p.ell * p.cv * sizeof(V);
+    const Div32 dcv = Div32::make((uint32_t)p.cv);
+    for (uint64_t s0 = 0; s0 < nslices; s0 += group) {
+        const uint64_t ns = std::min<uint64_t>(group, nslices - s0);
+        V* d = static_cast<V*>(data) + s0 * slice_vecs;
+        if (m.nheads) {
+            const uint64_t save_vecs = ns * m.nheads * (uint64_t)p.cv;
+            const int gs = (int)std::min<uint64_t>((save_vecs + kChunkBlock * 4 - 1) / (kChunkBlock * 4),
+                                                   (uint64_t)ncu_of(ctx) * 8);
+            hipLaunchKernelGGL(k_chunk_save<V>, dim3(gs), dim3(kChunkBlock), 0, st, (const V*)d,
+                               (V*)head, (const uint32_t*)m.hpos, m.nheads, ns, slice_vecs,
+                               (uint32_t)p.cv, dcv);
+            IPT_LAUNCH_CHECK("k_chunk_save");
+        }
+        const int gc = (int)std::min<uint64_t>(ns * m.nitems, (uint64_t)ncu_of(ctx) * 8);
+        hipLaunchKernelGGL(k_chunk_cycle<V>, dim3(gc), dim3(kChunkBlock), lds, st, d, (const V*)head,
+                           (const uint2*)m.items, (const uint32_t*)m.msrc, (const uint32_t*)m.mdst,
+                           m.nitems, m.nheads, ns, slice_vecs, (uint32_t)p.cv, dcv);
+        IPT_LAUNCH_CHECK("k_chunk_cycle");
     }
     return IPT_OK;
 }
@@ -344,27 +394,12 @@ ipt_status chunk_step_launch(const DeviceCtx* ctx, void* data, uint64_t nslices,
                              uint64_t slice_vecs, const BlockCyclePlan& p, const ChunkMoves& m,
                              void* head, uint64_t group, hipStream_t st) {
     if (m.nitems == 0) return IPT_OK;
-    const size_t lds = (size_t)p.ell * p.cv * 16;
-    const Div32 dcv = Div32::make((uint32_t)p.cv);
-    for (uint64_t s0 = 0; s0 < nslices; s0 += group) {
-        const uint64_t ns = std::min<uint64_t>(group, nslices - s0);
-        uint4* d = static_cast<uint4*>(data) + s0 * slice_vecs;
-        if (m.nheads) {
-            const uint64_t save_vecs = ns * m.nheads * (uint64_t)p.cv;
-            const int gs = (int)std::min<uint64_t>((save_vecs + kChunkBlock * 4 - 1) / (kChunkBlock * 4),
-                                                   (uint64_t)ncu_of(ctx) * 8);
-            hipLaunchKernelGGL(k_chunk_save, dim3(gs), dim3(kChunkBlock), 0, st, (const uint4*)d,
-                               (uint4*)head, (const uint32_t*)m.hpos, m.nheads, ns, slice_vecs,
-                               (uint32_t)p.cv, dcv);
-            IPT_LAUNCH_CHECK("k_chunk_save");
-        }
-        const int gc = (int)std::min<uint64_t>(ns * m.nitems, (uint64_t)ncu_of(ctx) * 8);
-        hipLaunchKernelGGL(k_chunk_cycle, dim3(gc), dim3(kChunkBlock), lds, st, d, (const uint4*)head,
-                           (const uint2*)m.items, (const uint32_t*)m.msrc, (const uint32_t*)m.mdst,
-                           m.nitems, m.nheads, ns, slice_vecs, (uint32_t)p.cv, dcv);
-        IPT_LAUNCH_CHECK("k_chunk_cycle");
+    switch (p.vb) {
+    case 16: return chunk_step_launch_v<uint4>(ctx, data, nslices, slice_vecs, p, m, head, group, st);
+    case 8: return chunk_step_launch_v<uint2>(ctx, data, nslices, slice_vecs, p, m, head, group, st);
+    case 4: return chunk_step_launch_v<uint32_t>(ctx, data, nslices, slice_vecs, p, m, head, group, st);
+    default: return IPT_ERR_INVALID_ARG;
     }
-    return IPT_OK;
 }
 
 // The chunk step of the per-call path: cached move lists, heads saved in the
@@ -389,7 +424,7 @@ static ipt_status chunk_step(DeviceCtx* ctx, void* data, const Shape& s, const B
                             &head);
         if (stt != IPT_OK) return stt;
     }
-    const uint64_t slice_vecs = (uint64_t)s.d1 * s.d2 * s.esz / 16;
+    const uint64_t slice_vecs = (uint64_t)s.d1 * s.d2 * s.esz / (uint64_t)p.vb;
     stt = chunk_step_launch(ctx, data, (uint64_t)s.d3, slice_vecs, p, m, head, group, st);
     if (!heads_per_slice) return stt;
     const ipt_status r = arena_release(ctx->scratch, st);
@@ -398,8 +433,8 @@ static ipt_status chunk_step(DeviceCtx* ctx, void* data, const Shape& s, const B
 
 ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,
                            hipStream_t st) {
-    const BlockCyclePlan p = block_cycle_plan(ctx, s);
-    if (!p.ok || (uintptr_t)data % 16 != 0) return IPT_ERR_UNSUPPORTED;
+    const BlockCyclePlan p = block_cycle_for(block_cycle_plan(ctx, s), data);
+    if (!p.ok) return IPT_ERR_UNSUPPORTED;
     if (p.chunk_first) {
         const ipt_status stt = chunk_step(ctx, data, s, p, st);
         return stt == IPT_OK ? run_variant(ctx, data, p.blk, p.block_variant, t, st) : stt;

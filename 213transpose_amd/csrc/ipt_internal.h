@@ -142,12 +142,15 @@ struct BlockCyclePlan {
     int64_t nb = 0;            // blocks (chunks along the chunked side) per slice
     int64_t other = 0;         // the side that is not chunked
     int64_t rc = 0, cc = 0;    // chunk matrix transposed by the chunk step
-    int64_t cv = 0;            // 16-byte vectors per chunk
+    int64_t vb = 16;           // bytes per vector the chunk step moves (16, 8 or 4)
+    int64_t cv = 0;            // vectors per chunk
     int64_t ell = 0;           // chunks per segment (at most)
     Shape blk;                 // the block step as a 213 problem
     int block_variant = 0;     // its one-sweep kernel
 };
 BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s);
+// The plan for data at `data`: vectors narrowed to the data's alignment.
+BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data);
 ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,
                            hipStream_t st);
 
@@ -168,9 +171,9 @@ void chunk_moves_free(ChunkMoves* m);
 ipt_status chunk_kernels_setup(const DeviceCtx* ctx);
 // Saved-head bytes per slice.
 inline uint64_t chunk_head_bytes(const BlockCyclePlan& p, const ChunkMoves& m) {
-    return (uint64_t)m.nheads * (uint64_t)p.cv * 16;
+    return (uint64_t)m.nheads * (uint64_t)p.cv * (uint64_t)p.vb;
 }
-// Chunk step over `nslices` slices of `slice_vecs` 16-byte vectors at data,
+// Chunk step over `nslices` slices of `slice_vecs` vectors (p.vb bytes) at data,
 // `group` slices at a time through `head` (group * chunk_head_bytes bytes).
 // Launches only: nothing is allocated, so it may be captured.
 ipt_status chunk_step_launch(const DeviceCtx* ctx, void* data, uint64_t nslices,

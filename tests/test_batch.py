@@ -99,7 +99,7 @@ def test_batch_block_cycle_instances(ipt, oracle):
     BC, AUTO = ipt.VARIANT_BLOCK_CYCLE, ipt.VARIANT_AUTO
     cases = [((65536, 8, 2, 4), BC), ((8, 65536, 1, 8), BC), ((6144, 2048, 1, 4), BC),
              ((2048, 6144, 1, 4), BC), ((4096, 96, 3, 8), BC), ((96, 4096, 2, 4), AUTO),
-             ((1001, 3, 1, 4), BC), ((300, 200, 1, 4), AUTO)]
+             ((1009, 3, 1, 4), BC), ((300, 200, 1, 4), AUTO)]
     buf, items, host = make_instances(oracle, [c for c, _ in cases], 1200)
     b = ipt.Batch([it + (v,) for it, (_, v) in zip(items, cases)])
     info = b.info()

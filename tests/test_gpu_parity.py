@@ -76,6 +76,8 @@ BLOCK_CYCLE_SHAPES = [  # (d1, d2, d3): skinny wide / tall, then large rectangle
     (1000, 3, 2), (3, 1000, 2), (6000, 100, 1), (100, 6000, 1),
     (6144, 4096, 1), (4096, 6144, 1), (5000, 3000, 2), (3000, 5000, 1), (8192, 2048, 3),
     (10007, 2000, 1), (2000, 10007, 1), (1500, 1000, 3),
+    # chunk sizes that are not multiples of 16 bytes (odd T): 8- and 4-byte vectors
+    (3003, 1001, 2), (1001, 3003, 1), (777, 4095, 1), (3751, 3723, 1),
 ]
 
 
@@ -95,20 +97,23 @@ def test_block_cycle_path_matches_oracle(ipt, oracle, esz):
     assert ran >= 16
 
 
-def test_block_cycle_unaligned_data_falls_back(ipt, oracle):
-    """The block-cycle path needs 16-byte aligned data: forced, it reports
-    unsupported; AUTO takes the decomposition instead."""
+def test_block_cycle_unaligned_data(ipt, oracle):
+    """Data one element past a 16-byte boundary: the chunk step moves 4-byte
+    vectors instead of 16-byte ones; forced and AUTO both exact."""
     d1, d2, d3, esz = 7, 1 << 22, 1, 4  # beyond GCS: AUTO resolves to the block-cycle path
     a = oracle.random_bits(d1 * d2 * d3 + 1, esz, 990)
     t = to_dev(a)
     view = t[1:]  # 4-byte offset
-    with pytest.raises(ipt.TransposeError):
-        ipt.transpose_variant(view, d1, d2, d3, ipt.VARIANT_BLOCK_CYCLE)
-    ipt.transpose(view, d1, d2, d3)
+    ipt.transpose_variant(view, d1, d2, d3, ipt.VARIANT_BLOCK_CYCLE)
     torch.cuda.synchronize()
     got = to_host(t, esz)
     assert got[0] == a[0]
-    assert np.array_equal(got[1:], oracle.seq_tt_213(a[1:], d1, d2, d3))
+    exp = oracle.seq_tt_213(a[1:], d1, d2, d3)
+    assert np.array_equal(got[1:], exp)
+    ipt.transpose(view, d2, d1, d3)  # AUTO, back
+    torch.cuda.synchronize()
+    got = to_host(t, esz)
+    assert np.array_equal(got, a)
 
 
 @pytest.mark.parametrize("esz", [4, 8])
