@@ -366,6 +366,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
                     big_tune = resolve_tune(ctx, &kCandidates[k].tune);
             }
         }
+        big_tune.opt_scratch = false;  // a captured batch cannot grow the scratch arena
         const Route p = plan_instance(ctx, s, forced, lds_cap, grid, b->tune, &big_variant);
         if (p == Route::Skip) {
             b->info.skipped++;

@@ -307,6 +307,17 @@ static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t
 // ---------------------------------------------------------------- passes
 
 // Column gather pass new[i][j] = old[g(i,j)][j] over every slice.
+// Tiers: a strip of W columns x all R rows in LDS (one sweep), else the
+// strip gathered into scratch and copied back (two sweeps).  The LDS strip
+// narrows as R grows; below 64 bytes of row segment its reads waste most of
+// every line, so the pass takes the scratch tier instead, with 256-byte
+// strips: a map whose source row is constant across runs of columns (a
+// rotation, Map::kRunConstant) gathers row segments directly; the two
+// diagonal maps (Map::kDiag) are walked in source order so their diagonal
+// side stays in L2 (k_col_diag_to_scratch).  Measured on f32 R=28876 x
+// C=9286 (strips of one column in LDS): C2R 13.9 -> 8.7 ms with the first
+// alone.  Not under stream capture (the arena cannot grow there) and not
+// where the caller rules scratch out (batches).
 template <typename T, typename Ix, typename Map>
 static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostPlan& h,
                          uint64_t d3, const Tune& t, hipStream_t st) {
@@ -322,36 +333,61 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
         int64_t W = cap;
         while (W > 1 && R * W * esz > t.lds_bytes) W /= 2;
         while (W > 1 && R * W * esz > lds_max) W /= 2;
-        const uint32_t nstrips = (uint32_t)((C + W - 1) / W);
-        const size_t lds = (size_t)(R * W * esz);
-        auto k = k_col_gather_lds<T, Ix, Map>;
-        ipt_status stt = allow_lds(ctx, k, lds);
-        if (stt != IPT_OK) return stt;
-        int grid;
-        grid_for(ctx, k, t.block, lds, d3 * nstrips, t.grid_mult, &grid);
-        hipLaunchKernelGGL(k, dim3(grid), dim3(t.block), lds, st, data, p, d3, (uint32_t)W, nstrips);
-        IPT_LAUNCH_CHECK("k_col_gather_lds");
-        return IPT_OK;
+        bool by_choice = false;
+        if (t.opt_scratch && W < C && W * esz < 64) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            by_choice = hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
+        }
+        if (!by_choice) {
+            const uint32_t nstrips = (uint32_t)((C + W - 1) / W);
+            const size_t lds = (size_t)(R * W * esz);
+            auto k = k_col_gather_lds<T, Ix, Map>;
+            ipt_status stt = allow_lds(ctx, k, lds);
+            if (stt != IPT_OK) return stt;
+            int grid;
+            grid_for(ctx, k, t.block, lds, d3 * nstrips, t.grid_mult, &grid);
+            hipLaunchKernelGGL(k, dim3(grid), dim3(t.block), lds, st, data, p, d3, (uint32_t)W, nstrips);
+            IPT_LAUNCH_CHECK("k_col_gather_lds");
+            return IPT_OK;
+        }
     }
-    // Columns taller than LDS: gather strips into scratch, copy back.
-    const uint32_t W = sizeof(T) == 4 ? 16 : 8;
-    const uint32_t nstrips = (uint32_t)((C + W - 1) / W);
+    // Scratch tier: gather strips into scratch, copy back.  Wide strips pay
+    // where a strip's row segment is contiguous on one side (every map but
+    // a general one; all four column maps here are such).
+    const uint32_t W = (Map::kRunConstant || Map::kDiag) ? (uint32_t)(256 / esz)
+                                                          : (sizeof(T) == 4 ? 16u : 8u);
+    const uint32_t wshift = (uint32_t)ceil_log2(W);
+    const uint64_t nstrips = ((uint64_t)C + W - 1) / W;
     const uint64_t items = d3 * nstrips;
-    const uint64_t per_item = (uint64_t)R * W * esz;
-    const uint64_t chunk = std::max<uint64_t>(1, kScratchCap / per_item);
+    const uint64_t per_item = (uint64_t)R * W;
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(kScratchCap / esz, (uint64_t)INT32_MAX) / per_item);
+    if (per_item > (uint64_t)INT32_MAX) return IPT_ERR_TOO_LARGE;  // one strip beyond 32-bit element math
+    const Div64 dstrips = Div64::make(nstrips);
+    const Div32 dper = Div32::make((uint32_t)per_item);
     std::lock_guard<std::mutex> g(ctx->mu);
     void* scr;
-    ipt_status stt = arena_acquire(ctx->scratch, (size_t)(std::min(chunk, items) * per_item), st, &scr);
+    ipt_status stt = arena_acquire(ctx->scratch, (size_t)(std::min(chunk, items) * per_item * esz), st, &scr);
     if (stt != IPT_OK) return stt;
     for (uint64_t first = 0; first < items; first += chunk) {
         const uint64_t n = std::min(chunk, items - first);
-        const uint64_t elems = n * R * W;
-        const int grid = (int)std::min<uint64_t>((elems + 255) / 256, (uint64_t)ncu_of(ctx) * 16);
-        hipLaunchKernelGGL((k_col_gather_to_scratch<T, Ix, Map>), dim3(grid), dim3(256), 0, st,
-                           data, (T*)scr, p, W, nstrips, first, n);
-        IPT_LAUNCH_CHECK("k_col_gather_to_scratch");
+        const uint32_t total = (uint32_t)(n * per_item);
+        const int grid = (int)std::min<uint64_t>(((uint64_t)total + 255) / 256, (uint64_t)ncu_of(ctx) * 16);
+        if constexpr (Map::kDiag != 0) {
+            constexpr uint32_t KB = 256;  // k rows per unit
+            const uint32_t nkb = (uint32_t)(((uint64_t)R + KB - 1) / KB);
+            const uint64_t units = n * nkb;
+            const int gd = (int)std::min<uint64_t>(units, (uint64_t)ncu_of(ctx) * 8);
+            hipLaunchKernelGGL((k_col_diag_to_scratch<T, Ix, Map::kDiag == 1>), dim3(gd), dim3(256), 0,
+                               st, data, (T*)scr, p, wshift, dstrips, first, (uint32_t)n, KB,
+                               Div32::make(nkb));
+            IPT_LAUNCH_CHECK("k_col_diag_to_scratch");
+        } else {
+            hipLaunchKernelGGL((k_col_gather_to_scratch<T, Ix, Map>), dim3(grid), dim3(256), 0, st,
+                               data, (T*)scr, p, wshift, dstrips, first, total, dper);
+            IPT_LAUNCH_CHECK("k_col_gather_to_scratch");
+        }
         hipLaunchKernelGGL((k_col_scratch_back<T>), dim3(grid), dim3(256), 0, st, data,
-                           (const T*)scr, (uint64_t)R, (uint64_t)C, W, nstrips, first, n);
+                           (const T*)scr, (uint64_t)R, (uint64_t)C, wshift, dstrips, first, total, dper);
         IPT_LAUNCH_CHECK("k_col_scratch_back");
     }
     return arena_release(ctx->scratch, st);

@@ -51,6 +51,7 @@ struct Tune {
     int block;      // threads per workgroup
     int strip;      // max column-strip width (elements)
     int grid_mult;  // workgroups per CU (x occupancy) for grid-stride kernels
+    bool opt_scratch = true;  // column passes may take the scratch tier by choice
 };
 
 constexpr int64_t kDefaultLds = 160 * 1024;       // gfx950 LDS per workgroup

@@ -314,29 +314,33 @@ k_col_gather_lds(T* __restrict__ data, Plan<Ix> p, uint64_t d3, uint32_t W, uint
 // the barrier between the last read of old data and the first overwrite.
 
 // scratch[(w-first)][i][jj] = old[g(i,j)][j] for strips w in [first, first+n).
+// Element e of the chunk (total < 2^31, so 32-bit reduced division) is strip
+// e / (R*W), row (e mod R*W) >> wshift, lane jj = e & (W - 1): consecutive
+// lanes walk one row segment of W columns, so a map whose source row is the
+// same across the strip (a rotation by a per-run constant) reads W*esz
+// contiguous bytes.  U elements per thread in flight; lanes past the end
+// repeat the last element, and columns past C (a partial last strip) read
+// column C - 1 into scratch slots the copy back never reads.
 template <typename T, typename Ix, typename Map>
 __global__ void __launch_bounds__(256)
 k_col_gather_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<Ix> p,
-                        uint32_t W, uint32_t nstrips, uint64_t first, uint64_t n) {
-    // U elements per thread in flight; lanes past the end repeat the last
-    // element, and columns past C (a partial last strip) read column C - 1
-    // into scratch slots the copy back never reads.
-    constexpr uint32_t U = 4;
-    const uint64_t R = p.R, C = p.C;
-    const uint64_t per_item = R * W;
-    const uint64_t total = n * per_item;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t e0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < total; e0 += stride * U) {
+                        uint32_t wshift, Div64 dstrips, uint64_t first, uint32_t total,
+                        Div32 dper) {
+    constexpr uint32_t U = 8;
+    const uint64_t C = p.C;
+    const uint32_t W = 1u << wshift;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t e0 = blockIdx.x * blockDim.x + threadIdx.x; e0 < total; e0 += stride * U) {
         T r[U];
-        uint64_t at[U];
+        uint32_t at[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint64_t e = umin64(e0 + u * stride, total - 1);
-            const uint64_t wl = e / per_item, rem = e - wl * per_item;
-            const uint64_t i = rem / W, jj = rem - i * W;
-            const uint64_t w = first + wl, sl = w / nstrips;
-            const uint64_t j = umin64((w - sl * nstrips) * W + jj, C - 1);
-            const T* base = data + sl * R * C;
+            const uint32_t e = min(e0 + u * stride, total - 1);
+            const uint32_t wl = dper.div(e), rem = e - wl * dper.d;
+            const uint32_t i = rem >> wshift, jj = rem & (W - 1);
+            const uint64_t w = first + wl, sl = dstrips.div(w);
+            const uint64_t j = umin64((w - sl * dstrips.d) * W + jj, C - 1);
+            const T* base = data + sl * (uint64_t)p.R * C;
             at[u] = e;
             r[u] = base[(uint64_t)Map::src(p, (Ix)i, (Ix)j) * C + j];
         }
@@ -345,25 +349,78 @@ k_col_gather_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Pla
     }
 }
 
+// The two diagonal column maps through scratch, in source order.  Both
+// factor through p = r2c_prepermute (the C2R post row term h(i) =
+// (i*C - i/a) mod R is p's inverse):
+//   C2R post  new[i][j] = old[(h(i) + j) mod R][j]: for k = h(i), row p(k)
+//             of the strip takes old[(k + j) mod R][j];
+//   R2C pre   new[i][j] = old[p((i - j) mod R)][j]: source row p(k) goes to
+//             rows (k + j) mod R.
+// A workgroup walks KB consecutive k of one strip, so the diagonal side (the
+// reads of the first, the writes of the second) slides one row per k over a
+// band of W + KB rows that stays in L2, and the other side moves whole
+// W-wide row segments.  Unit u = (strip item u / nkb, k block u mod nkb).
+template <typename T, typename Ix, bool C2R_POST>
+__global__ void __launch_bounds__(256)
+k_col_diag_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<Ix> p,
+                      uint32_t wshift, Div64 dstrips, uint64_t first, uint32_t n, uint32_t kb,
+                      Div32 dnkb) {
+    constexpr uint32_t U = 4;  // k rows per thread in flight
+    const uint64_t R = p.R, C = p.C;
+    const uint32_t W = 1u << wshift;
+    const uint32_t rows_per_pass = 256u >> wshift;  // k rows one pass of the workgroup covers
+    const uint32_t jj = threadIdx.x & (W - 1), kr = threadIdx.x >> wshift;
+    const uint32_t units = n * dnkb.d;
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint32_t wl = dnkb.div(u), kblk = u - wl * dnkb.d;
+        const uint64_t w = first + wl, sl = dstrips.div(w);
+        const uint64_t j = umin64((w - sl * dstrips.d) * W + jj, C - 1);
+        const T* base = data + sl * R * C;
+        T* out = scratch + (uint64_t)wl * R * W + jj;
+        const uint32_t k0 = kblk * kb, k1 = (uint32_t)umin64((uint64_t)k0 + kb, R);
+        const uint32_t jm = (uint32_t)(j % R);
+        for (uint32_t kk = k0 + kr; kk < k1; kk += rows_per_pass * U) {
+            T r[U];
+            uint32_t dst[U];
+#pragma unroll
+            for (uint32_t v = 0; v < U; v++) {
+                const uint32_t k = min(kk + v * rows_per_pass, k1 - 1);  // past the end: repeat the last row
+                uint32_t src;
+                if (C2R_POST) {
+                    src = k + jm;
+                    if (src >= R) src -= (uint32_t)R;
+                    dst[v] = (uint32_t)r2c_prepermute(p, (Ix)k);
+                } else {
+                    src = (uint32_t)r2c_prepermute(p, (Ix)k);
+                    dst[v] = k + jm;
+                    if (dst[v] >= R) dst[v] -= (uint32_t)R;
+                }
+                r[v] = base[(uint64_t)src * C + j];
+            }
+#pragma unroll
+            for (uint32_t v = 0; v < U; v++) out[(uint64_t)dst[v] * W] = r[v];
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_col_scratch_back(T* __restrict__ data, const T* __restrict__ scratch, uint64_t R, uint64_t C,
-                   uint32_t W, uint32_t nstrips, uint64_t first, uint64_t n) {
-    constexpr uint32_t U = 4;  // elements per thread in flight
-    const uint64_t per_item = R * W;
-    const uint64_t total = n * per_item;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t e0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < total; e0 += stride * U) {
+                   uint32_t wshift, Div64 dstrips, uint64_t first, uint32_t total, Div32 dper) {
+    constexpr uint32_t U = 8;  // elements per thread in flight
+    const uint32_t W = 1u << wshift;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t e0 = blockIdx.x * blockDim.x + threadIdx.x; e0 < total; e0 += stride * U) {
         T r[U];
         uint64_t dst[U];
         bool ok[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint64_t e = umin64(e0 + u * stride, total - 1);
-            const uint64_t wl = e / per_item, rem = e - wl * per_item;
-            const uint64_t i = rem / W, jj = rem - i * W;
-            const uint64_t w = first + wl, sl = w / nstrips;
-            const uint64_t j = (w - sl * nstrips) * W + jj;
+            const uint32_t e = min(e0 + u * stride, total - 1);
+            const uint32_t wl = dper.div(e), rem = e - wl * dper.d;
+            const uint32_t i = rem >> wshift, jj = rem & (W - 1);
+            const uint64_t w = first + wl, sl = dstrips.div(w);
+            const uint64_t j = (w - sl * dstrips.d) * W + jj;
             ok[u] = j < C;
             dst[u] = sl * R * C + i * C + j;
             r[u] = scratch[e];

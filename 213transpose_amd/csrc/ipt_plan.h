@@ -148,6 +148,8 @@ template <typename Ix> struct Plan {
 // c2r prerotator as a gather: (i + j/b) mod R (include/equations.h:78-94,
 // include/skinny.h:12-26).  i + j/b < R + c <= 2R.
 struct MapC2RPre {
+    static constexpr bool kRunConstant = true;  // same source row across runs of b columns
+    static constexpr int kDiag = 0;             // 1: C2R post form, 2: R2C pre form
     template <typename Ix> static IPT_HD Ix src(const Plan<Ix>& p, Ix i, Ix j) {
         Ix x = i + p.db.div(j);
         return x >= p.R ? x - p.R : x;
@@ -158,6 +160,8 @@ struct MapC2RPre {
 // (include/skinny.h:69-78 as used at src/skinny.cu:332; the large-path call
 // at src/transpose.cu:22 passes d2/c instead of c and is not followed).
 struct MapC2RPost {
+    static constexpr bool kRunConstant = false;
+    static constexpr int kDiag = 1;
     template <typename Ix> static IPT_HD Ix row_term(const Plan<Ix>& p, Ix i) {
         return p.dR.mod(i * p.C - p.da.div(i));
     }
@@ -177,6 +181,8 @@ template <typename Ix> IPT_HD Ix r2c_prepermute(const Plan<Ix>& p, Ix x) {
 
 // r2c col_op, fused form: p((i - j) mod R) (include/skinny.h:87-107).
 struct MapR2CPre {
+    static constexpr bool kRunConstant = false;
+    static constexpr int kDiag = 2;
     template <typename Ix> static IPT_HD Ix src(const Plan<Ix>& p, Ix i, Ix j) {
         Ix x = i + p.R - p.dR.mod(j);  // in [1, 2R)
         if (x >= p.R) x -= p.R;
@@ -187,6 +193,8 @@ struct MapR2CPre {
 // r2c postrotator as a gather: (i + (R - j/b) mod R) mod R
 // (include/equations.h:196-222, include/skinny.h:109-117).  j/b < c <= R.
 struct MapR2CPost {
+    static constexpr bool kRunConstant = true;
+    static constexpr int kDiag = 0;
     template <typename Ix> static IPT_HD Ix src(const Plan<Ix>& p, Ix i, Ix j) {
         Ix bd = p.db.div(j);
         Ix x = bd == 0 ? i : i + p.R - bd;

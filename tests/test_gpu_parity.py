@@ -193,7 +193,10 @@ def test_tune_knobs_do_not_change_results(ipt, oracle, esz):
 @pytest.mark.parametrize("esz", [4, 8])
 def test_each_pass_matches_oracle_pass(ipt, oracle, esz):
     """Per-pass parity: one GPU gather pass vs the oracle's same pass."""
-    cases = [(384, 512, 2), (256, 1000, 1), (1000, 256, 1), (96, 36, 3), (7, 4096, 1), (45, 45, 2)]
+    # (4096, 96) and (2500, 300): tall columns, so the rotation maps take the
+    # wide-strip scratch tier by choice
+    cases = [(384, 512, 2), (256, 1000, 1), (1000, 256, 1), (96, 36, 3), (7, 4096, 1), (45, 45, 2),
+             (4096, 96, 2), (2500, 300, 1)]
     for k, (R, C, d3) in enumerate(cases):
         for map_id in range(6):
             for tune in (None, ipt.Tune(lds_bytes=256)):
@@ -209,6 +212,16 @@ def test_each_pass_matches_oracle_pass(ipt, oracle, esz):
                 ipt.run_pass(t, R, C, d3, map_id, tune=tune)
                 torch.cuda.synchronize()
                 assert np.array_equal(to_host(t, esz), exp), (R, C, d3, map_id)
+
+
+@pytest.mark.parametrize("esz", [4, 8])
+def test_decomposition_tall_columns(ipt, oracle, esz):
+    """C2R and R2C on slices with tall columns and gcd > 1: the rotation
+    passes go through the wide-strip scratch tier, the others through narrow
+    LDS strips."""
+    for k, (d1, d2, d3) in enumerate([(96, 4096, 2), (4096, 96, 2), (300, 2500, 1), (2500, 300, 1)]):
+        for v in (ipt.VARIANT_C2R, ipt.VARIANT_R2C):
+            run_case(ipt, oracle, d1, d2, d3, esz, v, 1400 + 10 * k + v)
 
 
 def test_reference_entry_points(ipt, oracle):
