@@ -195,8 +195,9 @@ def test_each_pass_matches_oracle_pass(ipt, oracle, esz):
     """Per-pass parity: one GPU gather pass vs the oracle's same pass."""
     # (4096, 96) and (2500, 300): tall columns, so the rotation maps take the
     # wide-strip scratch tier by choice
+    # (1100, 1024, 16): many strips per slice, several slices
     cases = [(384, 512, 2), (256, 1000, 1), (1000, 256, 1), (96, 36, 3), (7, 4096, 1), (45, 45, 2),
-             (4096, 96, 2), (2500, 300, 1)]
+             (4096, 96, 2), (2500, 300, 1), (1100, 1024, 16)]
     for k, (R, C, d3) in enumerate(cases):
         for map_id in range(6):
             for tune in (None, ipt.Tune(lds_bytes=256)):
@@ -219,7 +220,8 @@ def test_decomposition_tall_columns(ipt, oracle, esz):
     """C2R and R2C on slices with tall columns and gcd > 1: the rotation
     passes go through the wide-strip scratch tier, the others through narrow
     LDS strips."""
-    for k, (d1, d2, d3) in enumerate([(96, 4096, 2), (4096, 96, 2), (300, 2500, 1), (2500, 300, 1)]):
+    for k, (d1, d2, d3) in enumerate([(96, 4096, 2), (4096, 96, 2), (300, 2500, 1), (2500, 300, 1),
+                                      (1100, 1024, 16)]):
         for v in (ipt.VARIANT_C2R, ipt.VARIANT_R2C):
             run_case(ipt, oracle, d1, d2, d3, esz, v, 1400 + 10 * k + v)
 
