@@ -61,7 +61,7 @@ k_chunk_save(const V* __restrict__ data, V* __restrict__ head, const uint32_t* _
              uint32_t nheads, uint64_t nslices, uint64_t slice_vecs, uint32_t cv, Div32 dcv) {
     const uint64_t per_slice = (uint64_t)nheads * cv;
     const uint64_t total = nslices * per_slice;
-    constexpr uint32_t U = 4;
+    constexpr uint32_t U = 4 * 16 / sizeof(V);  // 16 KB in flight per workgroup
     for (uint64_t e0 = (uint64_t)blockIdx.x * kChunkBlock * U; e0 < total;
          e0 += (uint64_t)gridDim.x * kChunkBlock * U) {
         V r[U];
@@ -92,7 +92,10 @@ k_chunk_cycle(V* __restrict__ data, const V* __restrict__ head, const uint2* __r
               uint32_t nheads, uint64_t nslices, uint64_t slice_vecs, uint32_t cv, Div32 dcv) {
     IPT_DYN_LDS(V, stage);
     __shared__ uint32_t src[kBcMaxSeg], dst[kBcMaxSeg];
-    constexpr uint32_t U = 8;
+    // 32 KB of loads in flight per workgroup whatever the vector width (with
+    // 8 loads of 4 bytes per thread the chunk step of an f32 1 GB slice ran
+    // at 3.1 TB/s)
+    constexpr uint32_t U = 8 * 16 / sizeof(V);
     const uint64_t total_items = nslices * nitems;
     for (uint64_t it = blockIdx.x; it < total_items; it += gridDim.x) {
         const uint64_t sl = it / nitems;
@@ -146,7 +149,9 @@ k_chunk_cycle(V* __restrict__ data, const V* __restrict__ head, const uint2* __r
 //               narrower vectors (x1.15 for 8 bytes, x1.4 for 4);
 //   block step  1 / efficiency of its kernel relative to a copy (measured:
 //               square swap ~1, whole-slice 16-byte form ~0.9, cooperative
-//               kernel ~0.85, whole-slice scalar form ~0.5).
+//               kernel ~0.95 with groups on one XCD, ~0.8 across XCDs, ~0.6
+//               with the largest register-tile class, whole-slice scalar
+//               form ~0.5).
 // The cheapest wins; ties go to the larger chunk.
 static int block_kernel(const DeviceCtx* ctx, const Shape& blk, double* eff) {
     const int64_t bytes = blk.d1 * blk.d2 * blk.esz;
@@ -154,7 +159,12 @@ static int block_kernel(const DeviceCtx* ctx, const Shape& blk, double* eff) {
     const GcsGeom gg = gcs_geom(ctx, blk, lds_max_of(ctx), true);
     const bool slice = bytes <= kBcLdsBlockBytes && variant_feasible(ctx, blk, IPT_VARIANT_SLICE_LDS);
     if (slice && (bytes % 16) == 0) { *eff = 0.9; return IPT_VARIANT_SLICE_LDS; }
-    if (gg.ok) { *eff = 0.85; return IPT_VARIANT_GCS; }
+    if (gg.ok) {
+        const int nx = std::max(ctx ? ctx->nxcd : 1, 1);
+        if (gcs_jr_class(blk.esz, gg.K, gg.KL, kGcsBlock / 64) == 2) *eff = 0.6;
+        else *eff = (int64_t)gg.g <= ncu_of(ctx) / nx ? 0.95 : 0.8;
+        return IPT_VARIANT_GCS;
+    }
     if (slice) { *eff = 0.5; return IPT_VARIANT_SLICE_LDS; }
     return IPT_VARIANT_AUTO;
 }
@@ -374,7 +384,8 @@ static ipt_status chunk_step_launch_v(const DeviceCtx* ctx, void* data, uint64_t
         V* d = static_cast<V*>(data) + s0 * slice_vecs;
         if (m.nheads) {
             const uint64_t save_vecs = ns * m.nheads * (uint64_t)p.cv;
-            const int gs = (int)std::min<uint64_t>((save_vecs + kChunkBlock * 4 - 1) / (kChunkBlock * 4),
+            const uint64_t per_wg = (uint64_t)kChunkBlock * 4 * 16 / sizeof(V);
+            const int gs = (int)std::min<uint64_t>((save_vecs + per_wg - 1) / per_wg,
                                                    (uint64_t)ncu_of(ctx) * 8);
             hipLaunchKernelGGL(k_chunk_save<V>, dim3(gs), dim3(kChunkBlock), 0, st, (const V*)d,
                                (V*)head, (const uint32_t*)m.hpos, m.nheads, ns, slice_vecs,

@@ -49,7 +49,10 @@ SHAPES = [  # (d1, d2, d3, esz): slice-in-LDS, GCS, square-swap and big routes, 
 @pytest.mark.gpu
 def test_batch_eager_and_graph_replays(ipt, oracle):
     buf, items, host = make_instances(oracle, SHAPES, 900)
-    b = ipt.Batch(items)
+    # AUTO everywhere except the 2048^2 f64 instance, forced onto the tile-pair
+    # route (which AUTO takes depends on the shipped selector's ranking)
+    v = [ipt.VARIANT_SQUARE_SWAP if s == (2048, 2048, 1, 8) else ipt.VARIANT_AUTO for s in SHAPES]
+    b = ipt.Batch([it + (vv,) for it, vv in zip(items, v)])
     info = b.info()
     print("[batch]", info, flush=True)
     assert info["n_slice_items"] > 0 and info["n_gcs_items"] > 0 and info["n_swap_items"] > 0
