@@ -89,16 +89,20 @@ typedef enum {
     IPT_VARIANT_C2R = 3,         /* C2R, view R = d2, C = d1 */
     IPT_VARIANT_R2C = 4,         /* R2C, view R = d1, C = d2 */
     IPT_VARIANT_GCS = 5,         /* group-cooperative staged: a group of workgroups
-                                    stages a slice in LDS, then writes it
-                                    transposed; one sweep for slices up to the
-                                    GPU's resident LDS */
+                                    holds a slice in LDS and registers, then
+                                    writes it transposed; one sweep for slices
+                                    up to ~109 MB, any row length and element
+                                    alignment */
     IPT_VARIANT_BLOCK_CYCLE = 6, /* one side cut into chunks of T elements:
                                     the chunks moved by segmented cycle
                                     following, blocks of T x (other side)
                                     transposed by a one-sweep kernel; about two
                                     sweeps.  With a short side and LDS-sized
                                     blocks it is the skinny path of
-                                    src/skinny.cu:313-363, rebuilt */
+                                    src/skinny.cu:313-363, rebuilt; it also
+                                    takes large rectangles.  Any element
+                                    alignment (chunks move as 16-, 8- or 4-byte
+                                    vectors) */
     IPT_VARIANT_COUNT = 7
 } ipt_variant;
 
