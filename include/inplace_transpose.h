@@ -243,7 +243,11 @@ int ipt_select_candidate(int elem_bytes, int64_t d1, int64_t d2, int64_t d3);
  * library scratch).
  * ipt_batch_run captures everything once into a hipGraph on the first call
  * and replays it afterwards.  Instance buffers must stay valid and must not
- * overlap; a batch must not run concurrently with itself. */
+ * overlap; a batch must not run concurrently with itself, nor with another
+ * batch or a cooperative (GCS) transpose on another stream: the persistent
+ * kernel's groups need every workgroup of its grid resident, and only
+ * stream-ordered calls into this library are ordered against it (an
+ * overlap is caught by the barrier watchdog, ipt_batch_check). */
 typedef struct {
     void* data;          /* device buffer of d1*d2*d3 elements */
     int32_t elem_bytes;  /* 4 or 8 */
