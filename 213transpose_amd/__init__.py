@@ -115,6 +115,8 @@ def lib():
     L.ipt_debug_run_pass.restype = _int
     L.ipt_debug_run_pass.argtypes = [_vp, _int, _i64, _i64, _i64, _int, ctypes.POINTER(Tune), _vp]
     L.ipt_debug_copy.restype = _int
+    L.ipt_debug_set_watchdog.restype = _int
+    L.ipt_debug_set_watchdog.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
     L.ipt_debug_copy.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
     L.ipt_device_check.restype = _int
     L.ipt_device_check.argtypes = []
@@ -476,6 +478,12 @@ def load_default_selector():
 def device_check():
     """Synchronise and raise if a cooperative group barrier gave up waiting."""
     _check(lib().ipt_device_check(), "ipt_device_check")
+
+
+def set_watchdog(polls=0, extra=0):
+    """Test hook (ipt_debug_set_watchdog): group barriers give up after
+    `polls` polls and wait for `extra` arrivals beyond their group."""
+    _check(lib().ipt_debug_set_watchdog(polls, extra), "ipt_debug_set_watchdog")
 
 
 def version():

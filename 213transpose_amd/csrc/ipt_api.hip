@@ -328,6 +328,24 @@ ipt_status ipt_device_check(void) {
     return IPT_OK;
 }
 
+ipt_status ipt_debug_set_watchdog(uint32_t polls, uint32_t extra) {
+    DeviceCtx* ctx;
+    ipt_status st = get_ctx(&ctx);
+    if (st != IPT_OK) return st;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    void* sync;
+    st = arena_acquire(ctx->sync, 32 * sizeof(uint32_t), nullptr, &sync);
+    if (st != IPT_OK) return st;
+    const uint32_t w[2] = {polls, extra};
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy((uint32_t*)sync + 3, w, sizeof(w), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error("ipt_debug_set_watchdog", e);
+        return IPT_ERR_HIP;
+    }
+    return IPT_OK;
+}
+
 const char* ipt_version(void) {
     static char buf[128];
     DeviceCtx* ctx;

@@ -478,6 +478,9 @@ namespace ipt {
 //   group_arrive_wait); no two slices share a line.
 // The spin is bounded: after ~2^26 polls the member records ctl[2] and
 // proceeds, so a broken invariant shows up as an error, never as a hang.
+// ctl[3] (a poll limit in place of 2^26) and ctl[4] (arrivals to wait for
+// beyond the group) are zero in use; a test sets them, through
+// ipt_debug_set_watchdog, to trip the watchdog on purpose.
 //
 // Deadlock freedom: the grid never exceeds the resident workgroups and
 // g <= grid.  A member of slice S waits only for the other members of S, and
@@ -498,11 +501,11 @@ IPT_HD void atomic_store_agent(uint32_t* p, uint32_t v) {
 constexpr uint32_t kSpinLimit = 1u << 26;
 
 // Wait until *ctr reaches target (thread 0 only).  Returns false on watchdog.
-__device__ __forceinline__ bool spin_until(uint32_t* ctr, uint32_t target) {
+__device__ __forceinline__ bool spin_until(uint32_t* ctr, uint32_t target, uint32_t limit) {
     uint32_t polls = 0;
     while (atomic_load_agent(ctr) < target) {
         __builtin_amdgcn_s_sleep(1);
-        if (++polls > kSpinLimit) return false;
+        if (++polls > limit) return false;
     }
     return true;
 }
@@ -523,16 +526,17 @@ constexpr uint32_t kBarLines = 17;
 // Words of barrier state per slice for a group of g.
 IPT_HD uint64_t bar_words(uint32_t g) { return 32ull * (1 + (g < kBarLines - 1 ? g : kBarLines - 1)); }
 
-__device__ __forceinline__ bool group_arrive_wait(uint32_t* bar, uint32_t m, uint32_t g) {
+__device__ __forceinline__ bool group_arrive_wait(uint32_t* bar, uint32_t m, uint32_t g,
+                                                  uint32_t extra, uint32_t limit) {
     if (g <= kBarLines - 1) {
         atomic_add_agent(&bar[0], 1u);
-        return spin_until(&bar[0], g);
+        return spin_until(&bar[0], g + extra, limit);
     }
     const uint32_t nsub = kBarLines - 1;
     const uint32_t q = m % nsub;
     const uint32_t members = g / nsub + (q < g % nsub ? 1u : 0u);
     if (atomic_add_agent(&bar[32 * (1 + q)], 1u) == members - 1) atomic_add_agent(&bar[0], 1u);
-    return spin_until(&bar[0], nsub);
+    return spin_until(&bar[0], nsub + extra, limit);
 }
 
 __device__ __forceinline__ void group_depart(uint32_t* bar, uint32_t g) {
@@ -551,14 +555,18 @@ __device__ __forceinline__ void group_sync(uint32_t* bar, uint32_t* ctl, uint32_
     wait_vmem();
     __syncthreads();
     if (g > 1) {
-        if (threadIdx.x == 0 && !group_arrive_wait(bar, m, g)) atomic_store_agent(&ctl[2], 1u);
+        if (threadIdx.x == 0) {
+            const uint32_t limit = atomic_load_agent(&ctl[3]), extra = atomic_load_agent(&ctl[4]);
+            if (!group_arrive_wait(bar, m, g, extra, limit ? limit : kSpinLimit))
+                atomic_store_agent(&ctl[2], 1u);
+        }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// Vector path (both dimensions multiples of 16 bytes, 16-byte aligned data,
-// slice below 2^31 bytes).
+// Wave-tile path (any row length and element alignment, slices below 2^31
+// bytes).
 //
 // Global accesses go through a raw buffer resource over the slice: a load at
 // an offset outside it returns zero without touching memory and a store

@@ -160,6 +160,14 @@ ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, 
  * calibration reference for bandwidth and for the HBM traffic counters. */
 ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream);
 
+/* Test hook for the cooperative kernels' barrier watchdog on the current
+ * device: every group barrier of later standalone launches gives up after
+ * `polls` polls (0: the default, ~2^26) and waits for `extra` arrivals beyond
+ * its group (0 in use), so a test can trip the watchdog without a real
+ * fault; ipt_device_check then reports it and clears both settings.
+ * Synchronises the device. */
+ipt_status ipt_debug_set_watchdog(uint32_t polls, uint32_t extra);
+
 /* ---- autotuner and learned selector ----------------------------------- */
 /* No counterpart in the reference: its tier thresholds are magic numbers
  * (src/shuffle.cu:15-50, src/rotate.cu:241-258).  Here IPT_NUM_CANDIDATES

@@ -169,6 +169,27 @@ def test_gcs_any_row_length_and_base(ipt, oracle, esz):
     assert ran >= 12
 
 
+def test_gcs_watchdog_reports_and_recovers(ipt, oracle):
+    """A barrier that cannot complete (one arrival more than the group) gives
+    up after a few polls: the kernel still ends, ipt_device_check reports
+    the watchdog and clears it, and the next transposes are exact."""
+    d1, d2, d3, esz = 512, 384, 8, 4
+    a = oracle.random_bits(d1 * d2 * d3, esz, 3100)
+    t = to_dev(a)
+    ipt.set_watchdog(2000, 1)
+    try:
+        ipt.transpose_variant(t, d1, d2, d3, ipt.VARIANT_GCS)
+        torch.cuda.synchronize()
+        with pytest.raises(ipt.TransposeError):
+            ipt.device_check()
+    finally:
+        ipt.set_watchdog(0, 0)
+    ipt.device_check()
+    for k, (e1, e2, e3, ez) in enumerate([(512, 384, 8, 4), (256, 1000, 3, 8)]):
+        run_case(ipt, oracle, e1, e2, e3, ez, ipt.VARIANT_GCS, 3110 + k)
+    ipt.device_check()
+
+
 @pytest.mark.parametrize("esz", [4, 8])
 def test_scratch_tiers_forced_by_small_lds_cap(ipt, oracle, esz):
     """lds_bytes=256 pushes every column pass with R*esz > 256 and every row
