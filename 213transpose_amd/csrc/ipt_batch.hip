@@ -136,8 +136,8 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
     default: break;
     }
     if (slice_bytes <= lds_cap) return Route::Slice;
+    if (s.d1 == s.d2) return Route::Swap;  // faster than the cooperative route on squares
     if (gcs_ok) return Route::Gcs;
-    if (s.d1 == s.d2) return Route::Swap;
     *big_variant = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
     (void)t;
     return Route::Big;
@@ -153,7 +153,7 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
 int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t lds_cap, int grid) {
     const int tuned = tuned_candidate(s);
     if (tuned >= 0) return tuned;
-    if (!y || !selector_domain(s.d1, s.d2, s.d3, s.esz)) return -1;
+    if (!y || !selector_domain(s.d1, s.d2, s.d3, s.esz) || s.d1 == s.d2) return -1;
     const int64_t slice_bytes = s.d2 * slice_pitch(s.d1) * s.esz;
     const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, kBatchJrClass);
     const bool route_ok[4] = {slice_bytes <= lds_cap, s.d1 == s.d2,
@@ -303,8 +303,9 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             b->info.n_gcs_items += (uint64_t)gg.g * s.d3;
         } else {
             const uint64_t TS = s.esz == 4 ? 64 : 32;
-            I.kind = ((uint64_t)s.d1 * s.esz) % 16 == 0 && (uintptr_t)data % 16 == 0 ? BK_SWAP_VEC
-                                                                                      : BK_SWAP;
+            const bool vec = ((uint64_t)s.d1 * s.esz) % 16 == 0 && (uintptr_t)data % 16 == 0;
+            const bool rs = (uint64_t)s.d1 * s.d1 * s.esz < ((uint64_t)1 << 31);
+            I.kind = vec ? (rs ? BK_SWAP_RS : BK_SWAP_VEC) : BK_SWAP;
             const uint64_t nt = ((uint64_t)s.d1 + TS - 1) / TS;
             const uint64_t pairs = nt * (nt + 1) / 2;
             if (pairs >= ((uint64_t)1 << 32)) return IPT_ERR_TOO_LARGE;

@@ -551,15 +551,27 @@ __device__ __forceinline__ void group_depart(uint32_t* bar, uint32_t g) {
 // Staging shared by the members of one slice: after every wave has its
 // loads back, thread 0 arrives and waits for the group; `bar` is null when
 // the slice has one member.
-__device__ __forceinline__ void group_sync(uint32_t* bar, uint32_t* ctl, uint32_t m, uint32_t g) {
+// The watchdog settings ctl[3], ctl[4], read by thread 0 before the member's
+// loads are issued (so the read overlaps them instead of delaying the
+// arrival; nothing writes them while a kernel runs).
+struct Watchdog {
+    uint32_t limit = kSpinLimit, extra = 0;
+    __device__ __forceinline__ void read(const uint32_t* ctl, uint32_t g) {
+        if (g > 1 && threadIdx.x == 0) {
+            const uint32_t l = ctl[3];
+            limit = l ? l : kSpinLimit;
+            extra = ctl[4];
+        }
+    }
+};
+
+__device__ __forceinline__ void group_sync(uint32_t* bar, uint32_t* ctl, uint32_t m, uint32_t g,
+                                           const Watchdog& wd) {
     wait_vmem();
     __syncthreads();
     if (g > 1) {
-        if (threadIdx.x == 0) {
-            const uint32_t limit = atomic_load_agent(&ctl[3]), extra = atomic_load_agent(&ctl[4]);
-            if (!group_arrive_wait(bar, m, g, extra, limit ? limit : kSpinLimit))
-                atomic_store_agent(&ctl[2], 1u);
-        }
+        if (threadIdx.x == 0 && !group_arrive_wait(bar, m, g, wd.extra, wd.limit))
+            atomic_store_agent(&ctl[2], 1u);
         __syncthreads();
     }
 }
@@ -805,6 +817,8 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     const uint32_t nk = m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0;
     const uint32_t kl = min(nk, KL);
     const Rsrc rs = make_rsrc(base, R * C * (uint32_t)sizeof(T));
+    Watchdog wd;
+    wd.read(ctl, g);
     auto origin = [&](uint32_t k, uint32_t* r0, uint32_t* c0) {
         const uint32_t t = m + k * g;
         const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
@@ -837,7 +851,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     }
 #pragma unroll
     for (int j = 0; j < JR; j++) rt[j].transpose();
-    group_sync(bar, ctl, m, g);
+    group_sync(bar, ctl, m, g, wd);
 #pragma unroll
     for (int j = 0; j < JL; j++) {
         const uint32_t k = wave + j * NW;
@@ -898,7 +912,7 @@ namespace ipt {
 // the earliest unfinished group completes; the frontier group gets its
 // remaining members from workgroups freed as earlier groups complete (all
 // grid workgroups would suffice).  Only group members ever wait.
-enum BatchKind : uint32_t { BK_SLICE = 0, BK_GCS = 1, BK_SWAP = 2, BK_SWAP_VEC = 3 };
+enum BatchKind : uint32_t { BK_SLICE = 0, BK_GCS = 1, BK_SWAP = 2, BK_SWAP_VEC = 3, BK_SWAP_RS = 4 };
 
 struct BatchInst {
     uint64_t data;        // device address of slice 0
@@ -945,6 +959,9 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
                                                               I.ntiles, it.sub, bar, ctl, l);
         break;
     }
+    case BK_SWAP_RS:  // buffer-resource form: slices below 2^31 bytes, 16-byte rows
+        swap_body_rs<T, TS_SWAP, kBatchBlock>(base, I.R, it.sub, lds);
+        break;
     case BK_SWAP_VEC:
         swap_body<T, TS_SWAP, kBatchBlock, true>(base, I.R, it.sub, lds);
         break;

@@ -142,7 +142,12 @@ static void load_default_weights_locked() {
 }
 
 int select_candidate(const DeviceCtx* ctx, const Shape& s) {
-    const bool in_domain = selector_domain(s.d1, s.d2, s.d3, s.esz);
+    // Square slices are rare in the sweep (m = n exactly), and the network
+    // ranks them poorly; they keep the heuristic's square swap (measured
+    // faster than every other candidate on every square tried) unless the
+    // autotuner recorded a choice.
+    const bool square = s.d1 == s.d2 && s.d1 * s.d2 * s.esz > 16 * 1024;
+    const bool in_domain = !square && selector_domain(s.d1, s.d2, s.d3, s.esz);
     {
         std::lock_guard<std::mutex> g(g_sel_mu);
         auto tuned = g_tuned.find(std::make_tuple(s.esz, s.d1, s.d2, s.d3));
