@@ -115,6 +115,8 @@ def lib():
     L.ipt_debug_run_pass.restype = _int
     L.ipt_debug_run_pass.argtypes = [_vp, _int, _i64, _i64, _i64, _int, ctypes.POINTER(Tune), _vp]
     L.ipt_debug_copy.restype = _int
+    L.ipt_permute3.restype = _int
+    L.ipt_permute3.argtypes = [_vp, _int, _i64, _i64, _i64, ctypes.POINTER(ctypes.c_int), _vp]
     L.ipt_debug_set_watchdog.restype = _int
     L.ipt_debug_set_watchdog.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
     L.ipt_debug_copy.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
@@ -212,6 +214,16 @@ def _r2c(data, d1, d2, d3=1, stream=None):
 c2r = types.SimpleNamespace(transpose=_c2r)
 # inplace::r2c::transpose: slices of d1 rows x d2 cols (reference argument order).
 r2c = types.SimpleNamespace(transpose=_r2c)
+
+
+def permute3(data, dims, perm, stream=None):
+    """ipt_permute3: any permutation of a rank-3 tensor in place (dims fastest
+    first; output dim j is source dim perm[j], as the oracle's seq_tt)."""
+    d1, d2, d3 = dims
+    ptr, esz = _args(data, d1, d2, d3)
+    p = (ctypes.c_int * 3)(*perm)
+    _check(lib().ipt_permute3(ptr, esz, d1, d2, d3, p, _stream_ptr(stream)), "ipt_permute3")
+    return data
 
 
 def openmp_style(row_major, data, m, n, stream=None):

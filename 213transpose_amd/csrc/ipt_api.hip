@@ -250,6 +250,65 @@ ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m, i
     return ipt_transpose_213(data, elem_bytes, m, n, 1, stream);
 }
 
+// Rank-3 permutations: the shortest word in the two adjacent swaps that
+// generate them, each swap one in-place pass over the tensor:
+//   swap of positions 0, 1  the 213 transpose of the current dims;
+//   swap of positions 1, 2  the outer two dims exchanged, the innermost kept
+//                           contiguous: a transpose of a matrix of chunks
+//                           of the innermost dim (as the 213 transpose of
+//                           4- or 8-byte super-elements when a chunk is one,
+//                           else the block-cycle chunk step).
+// Layout state: cur[j] = the source dim at position j (fastest first), the
+// same convention as the permutation argument.
+ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
+                        const int* perm, void* stream) {
+    ipt_status st = validate(data, elem_bytes, d1, d2, d3);
+    if (st != IPT_OK) return st;
+    if (!perm) return IPT_ERR_INVALID_ARG;
+    int seen[3] = {0, 0, 0};
+    for (int j = 0; j < 3; j++) {
+        if (perm[j] < 0 || perm[j] > 2 || seen[perm[j]]) return IPT_ERR_INVALID_ARG;
+        seen[perm[j]] = 1;
+    }
+    const int64_t dims[3] = {d1, d2, d3};
+    // Breadth-first over the six states; moves 0 (swap 0,1) and 1 (swap 1,2).
+    auto code = [](const int* c) { return c[0] * 9 + c[1] * 3 + c[2]; };
+    int prev[27], move[27], cur[3] = {0, 1, 2};
+    for (int i = 0; i < 27; i++) prev[i] = -2;
+    int queue[6][3], qh = 0, qt = 0;
+    prev[code(cur)] = -1;
+    std::copy(cur, cur + 3, queue[qt++]);
+    while (qh < qt) {
+        const int* c = queue[qh++];
+        for (int mv = 0; mv < 2; mv++) {
+            int n[3] = {c[0], c[1], c[2]};
+            std::swap(n[mv], n[mv + 1]);
+            if (prev[code(n)] != -2) continue;
+            prev[code(n)] = code(c);
+            move[code(n)] = mv;
+            std::copy(n, n + 3, queue[qt++]);
+        }
+    }
+    int path[3], len = 0;
+    for (int c = code(perm); prev[c] != -1; c = prev[c]) path[len++] = move[c];
+    DeviceCtx* ctx;
+    st = get_ctx(&ctx);
+    if (st != IPT_OK) return st;
+    for (int k = len - 1; k >= 0; k--) {
+        const int64_t a = dims[cur[0]], b = dims[cur[1]], c = dims[cur[2]];
+        if (path[k] == 0) {
+            st = ipt_transpose_213(data, elem_bytes, a, b, c, stream);
+        } else if (a * elem_bytes == 4 || (a * elem_bytes == 8 && (uintptr_t)data % 8 == 0)) {
+            st = ipt_transpose_213(data, (int)(a * elem_bytes), b, c, 1, stream);
+        } else {
+            st = run_chunk_transpose(ctx, data, elem_bytes, a, c, b, (hipStream_t)stream);
+        }
+        if (st != IPT_OK) return st;
+        std::swap(cur[path[k]], cur[path[k] + 1]);
+    }
+    return IPT_OK;
+}
+
 int ipt_variant_feasible(int variant, int elem_bytes, int64_t d1, int64_t d2, int64_t d3) {
     if (elem_bytes != 4 && elem_bytes != 8) return 0;
     if (d1 <= 0 || d2 <= 0 || d3 <= 0) return 0;

@@ -247,6 +247,40 @@ def test_decomposition_tall_columns(ipt, oracle, esz):
             run_case(ipt, oracle, d1, d2, d3, esz, v, 1400 + 10 * k + v)
 
 
+PERMS3 = [(0, 1, 2), (1, 0, 2), (0, 2, 1), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+@pytest.mark.parametrize("esz", [4, 8])
+def test_permute3_matches_oracle(ipt, oracle, esz):
+    """Every rank-3 permutation against the oracle's rank-N seq_tt: inner dims
+    of one element (super-element 213), two (8-byte super-elements for f32),
+    odd sizes (4- / 8-byte chunk vectors) and 16-byte multiples, several outer
+    shapes; and a base one element past a 16-byte boundary."""
+    shapes = [(5, 7, 9), (1, 50, 40), (2, 33, 31), (16, 33, 20), (3, 100, 64), (64, 48, 30),
+              (8, 2, 300), (300, 5, 7), (33, 17, 65)]
+    for k, dims in enumerate(shapes):
+        n = dims[0] * dims[1] * dims[2]
+        for perm in PERMS3:
+            for off in (0, 1) if k % 3 == 0 else (0,):
+                print(f"[case] permute3 dims={dims} perm={perm} esz={esz} offset={off}", flush=True)
+                a = oracle.random_bits(n + 1, esz, 3300 + 10 * k)
+                t = to_dev(a)
+                ipt.permute3(t[off:off + n], dims, perm)
+                torch.cuda.synchronize()
+                got = to_host(t, esz)
+                exp = a.copy()
+                exp[off:off + n] = oracle.seq_tt(a[off:off + n], list(dims), list(perm))
+                assert np.array_equal(got, exp), (dims, perm, esz, off)
+    ipt.device_check()
+
+
+def test_permute3_rejects_bad_perm(ipt):
+    t = torch.zeros(24, dtype=torch.int32, device="cuda")
+    for perm in [(0, 0, 1), (0, 1, 3), (-1, 1, 2)]:
+        with pytest.raises(ipt.TransposeError):
+            ipt.permute3(t, (2, 3, 4), perm)
+
+
 def test_reference_entry_points(ipt, oracle):
     d1, d2, d3 = 300, 100, 2
     a = oracle.random_bits(d1 * d2 * d3, 4, 7)
