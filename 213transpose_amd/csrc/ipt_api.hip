@@ -250,16 +250,18 @@ ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m, i
     return ipt_transpose_213(data, elem_bytes, m, n, 1, stream);
 }
 
-// Rank-3 permutations: the shortest word in the two adjacent swaps that
-// generate them, each swap one in-place pass over the tensor:
-//   swap of positions 0, 1  the 213 transpose of the current dims;
-//   swap of positions 1, 2  the outer two dims exchanged, the innermost kept
-//                           contiguous: a transpose of a matrix of chunks
-//                           of the innermost dim (as the 213 transpose of
-//                           4- or 8-byte super-elements when a chunk is one,
-//                           else the block-cycle chunk step).
-// Layout state: cur[j] = the source dim at position j (fastest first), the
-// same convention as the permutation argument.
+// Rank-3 permutations: the shortest sequence of four one-pass moves, each
+// an in-place transpose over the whole tensor (layout state: cur[j] = the
+// source dim at position j, fastest first, the convention of `perm`):
+//   0  (s0 s1 s2) -> (s1 s0 s2)  the 213 transpose of the current dims;
+//   1  (s0 s1 s2) -> (s0 s2 s1)  the outer two dims exchanged, the innermost
+//                                kept contiguous: a transpose of a matrix of
+//                                innermost-dim chunks (the 213 transpose of
+//                                4- or 8-byte super-elements when a chunk is
+//                                one, else the block-cycle chunk step);
+//   2  (s0 s1 s2) -> (s1 s2 s0)  the 213 transpose of (a, b*c, 1);
+//   3  (s0 s1 s2) -> (s2 s0 s1)  the 213 transpose of (a*b, c, 1).
+// Every permutation but the reversal is one move; the reversal is two.
 ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
                         const int* perm, void* stream) {
     ipt_status st = validate(data, elem_bytes, d1, d2, d3);
@@ -271,7 +273,11 @@ ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
         seen[perm[j]] = 1;
     }
     const int64_t dims[3] = {d1, d2, d3};
-    // Breadth-first over the six states; moves 0 (swap 0,1) and 1 (swap 1,2).
+    auto apply = [](int mv, const int* c, int* n) {
+        static const int order[4][3] = {{1, 0, 2}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
+        for (int j = 0; j < 3; j++) n[j] = c[order[mv][j]];
+    };
+    // Breadth-first over the six states.
     auto code = [](const int* c) { return c[0] * 9 + c[1] * 3 + c[2]; };
     int prev[27], move[27], cur[3] = {0, 1, 2};
     for (int i = 0; i < 27; i++) prev[i] = -2;
@@ -280,9 +286,9 @@ ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
     std::copy(cur, cur + 3, queue[qt++]);
     while (qh < qt) {
         const int* c = queue[qh++];
-        for (int mv = 0; mv < 2; mv++) {
-            int n[3] = {c[0], c[1], c[2]};
-            std::swap(n[mv], n[mv + 1]);
+        for (int mv = 0; mv < 4; mv++) {
+            int n[3];
+            apply(mv, c, n);
             if (prev[code(n)] != -2) continue;
             prev[code(n)] = code(c);
             move[code(n)] = mv;
@@ -296,15 +302,21 @@ ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
     if (st != IPT_OK) return st;
     for (int k = len - 1; k >= 0; k--) {
         const int64_t a = dims[cur[0]], b = dims[cur[1]], c = dims[cur[2]];
-        if (path[k] == 0) {
-            st = ipt_transpose_213(data, elem_bytes, a, b, c, stream);
-        } else if (a * elem_bytes == 4 || (a * elem_bytes == 8 && (uintptr_t)data % 8 == 0)) {
-            st = ipt_transpose_213(data, (int)(a * elem_bytes), b, c, 1, stream);
-        } else {
-            st = run_chunk_transpose(ctx, data, elem_bytes, a, c, b, (hipStream_t)stream);
+        switch (path[k]) {
+        case 0: st = ipt_transpose_213(data, elem_bytes, a, b, c, stream); break;
+        case 1:
+            if (a * elem_bytes == 4 || (a * elem_bytes == 8 && (uintptr_t)data % 8 == 0))
+                st = ipt_transpose_213(data, (int)(a * elem_bytes), b, c, 1, stream);
+            else
+                st = run_chunk_transpose(ctx, data, elem_bytes, a, c, b, (hipStream_t)stream);
+            break;
+        case 2: st = ipt_transpose_213(data, elem_bytes, a, b * c, 1, stream); break;
+        default: st = ipt_transpose_213(data, elem_bytes, a * b, c, 1, stream); break;
         }
         if (st != IPT_OK) return st;
-        std::swap(cur[path[k]], cur[path[k] + 1]);
+        int n[3];
+        apply(path[k], cur, n);
+        std::copy(n, n + 3, cur);
     }
     return IPT_OK;
 }

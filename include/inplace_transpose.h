@@ -85,10 +85,11 @@ ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m,
  * first, and output dim j is source dim perm[j]; {1,0,2} is the 213
  * transpose above.  The next step the reference's oracle points to (it
  * handles any rank and permutation); no in-place counterpart in the
- * reference.  Composed of at most three passes: 213 transposes and
- * exchanges of the outer two dims with the innermost kept contiguous (a
- * transpose of a matrix of innermost-dim chunks, chunks up to 64 KB;
- * IPT_ERR_UNSUPPORTED beyond). */
+ * reference.  One in-place pass for every permutation but the reversal
+ * {2,1,0}, which takes two: 213 transposes of the dims or of merged dims,
+ * and the exchange of the outer two dims with the innermost kept
+ * contiguous (a transpose of a matrix of innermost-dim chunks, chunks up to
+ * 64 KB; IPT_ERR_UNSUPPORTED beyond). */
 ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
                         const int* perm, void* stream);
 
