@@ -67,6 +67,15 @@ template <typename T> void transpose(T* data, int d1, int d2, int d3, void* stre
 }
 }  // namespace r2c
 
+// Any permutation of a rank-3 tensor in place: dims fastest first, output
+// dim j is source dim perm[j] (the oracle's seq_tt convention).  No
+// counterpart in the reference's in-place API.
+template <typename T>
+void permute3(T* data, int d1, int d2, int d3, const int (&perm)[3], void* stream = nullptr) {
+    detail::check(ipt_permute3(data, detail::elem_bytes<T>(), d1, d2, d3, perm, stream),
+                  "inplace::permute3");
+}
+
 namespace openmp {
 template <typename T>
 void transpose(bool row_major, T* data, int m, int n, T* /*tmp*/, void* stream = nullptr) {
