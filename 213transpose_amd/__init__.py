@@ -115,6 +115,8 @@ def lib():
     L.ipt_debug_run_pass.restype = _int
     L.ipt_debug_run_pass.argtypes = [_vp, _int, _i64, _i64, _i64, _int, ctypes.POINTER(Tune), _vp]
     L.ipt_debug_copy.restype = _int
+    L.ipt_permute.restype = _int
+    L.ipt_permute.argtypes = [_vp, _int, _int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), _vp]
     L.ipt_permute3.restype = _int
     L.ipt_permute3.argtypes = [_vp, _int, _i64, _i64, _i64, ctypes.POINTER(ctypes.c_int), _vp]
     L.ipt_debug_set_watchdog.restype = _int
@@ -223,6 +225,19 @@ def permute3(data, dims, perm, stream=None):
     ptr, esz = _args(data, d1, d2, d3)
     p = (ctypes.c_int * 3)(*perm)
     _check(lib().ipt_permute3(ptr, esz, d1, d2, d3, p, _stream_ptr(stream)), "ipt_permute3")
+    return data
+
+
+def permute(data, dims, perm, stream=None):
+    """ipt_permute: any permutation of a tensor of rank <= 6 in place (dims
+    fastest first; output dim j is source dim perm[j], as the oracle's seq_tt)."""
+    total = 1
+    for d in dims:
+        total *= d
+    ptr, esz = _args(data, total, 1, 1)
+    d = (ctypes.c_int64 * len(dims))(*dims)
+    p = (ctypes.c_int * len(perm))(*perm)
+    _check(lib().ipt_permute(ptr, esz, len(dims), d, p, _stream_ptr(stream)), "ipt_permute")
     return data
 
 

@@ -14,9 +14,11 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <vector>
 
 namespace ipt {
 
@@ -250,75 +252,112 @@ ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m, i
     return ipt_transpose_213(data, elem_bytes, m, n, 1, stream);
 }
 
-// Rank-3 permutations: the shortest sequence of four one-pass moves, each
-// an in-place transpose over the whole tensor (layout state: cur[j] = the
-// source dim at position j, fastest first, the convention of `perm`):
-//   0  (s0 s1 s2) -> (s1 s0 s2)  the 213 transpose of the current dims;
-//   1  (s0 s1 s2) -> (s0 s2 s1)  the outer two dims exchanged, the innermost
-//                                kept contiguous: a transpose of a matrix of
-//                                innermost-dim chunks (the 213 transpose of
-//                                4- or 8-byte super-elements when a chunk is
-//                                one, else the block-cycle chunk step);
-//   2  (s0 s1 s2) -> (s1 s2 s0)  the 213 transpose of (a, b*c, 1);
-//   3  (s0 s1 s2) -> (s2 s0 s1)  the 213 transpose of (a*b, c, 1).
-// Every permutation but the reversal is one move; the reversal is two.
-ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
-                        const int* perm, void* stream) {
-    ipt_status st = validate(data, elem_bytes, d1, d2, d3);
+// Permutations of rank N <= kMaxRank in place.  A move exchanges two
+// adjacent groups of positions B = [i, j) and C = [j, k) (fastest first),
+// the positions before (A) kept innermost and those after (D) outermost:
+// one in-place pass,
+//   A = 1            the 213 transpose of (|B|, |C|, |D|);
+//   |A|*esz = 4, 8   the same on super-elements of |A| elements;
+//   otherwise        a transpose of |D| matrices of |C| x |B| chunks of |A|
+//                    elements (the block-cycle chunk step; chunks up to
+//                    64 KB, IPT_ERR_UNSUPPORTED beyond).
+// A move with |B| = 1 or |C| = 1 moves no data.  The layout state (cur[p] =
+// the source dim at position p, the convention of `perm`) is searched
+// breadth-first, moving no data counting zero, for the fewest passes: rank
+// 3 needs one for every permutation but the reversal, which takes two.
+static constexpr int kMaxRank = 6;
+
+static ipt_status permute_move(DeviceCtx* ctx, void* data, int esz, const int64_t* sizes,
+                               int rank, const int* cur, int i, int j, int k, void* stream) {
+    int64_t g[4] = {1, 1, 1, 1};  // |A|, |B|, |C|, |D|
+    for (int p = 0; p < rank; p++) g[p < i ? 0 : p < j ? 1 : p < k ? 2 : 3] *= sizes[cur[p]];
+    if (g[1] == 1 || g[2] == 1) return IPT_OK;
+    if (g[0] == 1) return ipt_transpose_213(data, esz, g[1], g[2], g[3], stream);
+    const int64_t e = g[0] * esz;
+    if (e == 4 || (e == 8 && (uintptr_t)data % 8 == 0))
+        return ipt_transpose_213(data, (int)e, g[1], g[2], g[3], stream);
+    return run_chunk_transpose(ctx, data, esz, g[0], g[2], g[1], g[3], (hipStream_t)stream);
+}
+
+ipt_status ipt_permute(void* data, int elem_bytes, int rank, const int64_t* dims, const int* perm,
+                       void* stream) {
+    if (!data || !dims || !perm || rank < 1 || rank > kMaxRank) return IPT_ERR_INVALID_ARG;
+    int64_t total = 1;
+    int seen[kMaxRank] = {0};
+    for (int p = 0; p < rank; p++) {
+        if (dims[p] <= 0 || perm[p] < 0 || perm[p] >= rank || seen[perm[p]]) return IPT_ERR_INVALID_ARG;
+        seen[perm[p]] = 1;
+        if (total > ((int64_t)1 << 62) / dims[p]) return IPT_ERR_TOO_LARGE;
+        total *= dims[p];
+    }
+    ipt_status st = validate(data, elem_bytes, total, 1, 1);
     if (st != IPT_OK) return st;
-    if (!perm) return IPT_ERR_INVALID_ARG;
-    int seen[3] = {0, 0, 0};
-    for (int j = 0; j < 3; j++) {
-        if (perm[j] < 0 || perm[j] > 2 || seen[perm[j]]) return IPT_ERR_INVALID_ARG;
-        seen[perm[j]] = 1;
-    }
-    const int64_t dims[3] = {d1, d2, d3};
-    auto apply = [](int mv, const int* c, int* n) {
-        static const int order[4][3] = {{1, 0, 2}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}};
-        for (int j = 0; j < 3; j++) n[j] = c[order[mv][j]];
+    // States: permutations of rank positions, coded in base `rank`.
+    auto encode = [rank](const int* c) {
+        int v = 0;
+        for (int p = 0; p < rank; p++) v = v * rank + c[p];
+        return v;
     };
-    // Breadth-first over the six states.
-    auto code = [](const int* c) { return c[0] * 9 + c[1] * 3 + c[2]; };
-    int prev[27], move[27], cur[3] = {0, 1, 2};
-    for (int i = 0; i < 27; i++) prev[i] = -2;
-    int queue[6][3], qh = 0, qt = 0;
-    prev[code(cur)] = -1;
-    std::copy(cur, cur + 3, queue[qt++]);
-    while (qh < qt) {
-        const int* c = queue[qh++];
-        for (int mv = 0; mv < 4; mv++) {
-            int n[3];
-            apply(mv, c, n);
-            if (prev[code(n)] != -2) continue;
-            prev[code(n)] = code(c);
-            move[code(n)] = mv;
-            std::copy(n, n + 3, queue[qt++]);
-        }
+    int states = 1;
+    for (int p = 0; p < rank; p++) states *= rank;
+    std::vector<int> dist(states, -1), prev(states, -1), mi(states), mj(states), mk(states);
+    std::vector<std::vector<int>> cfg(states);
+    std::deque<int> dq;
+    std::vector<int> start(rank);
+    for (int p = 0; p < rank; p++) start[p] = p;
+    const int s0 = encode(start.data());
+    dist[s0] = 0;
+    cfg[s0] = start;
+    dq.push_back(s0);
+    while (!dq.empty()) {  // 0-1 breadth-first search
+        const int sc = dq.front();
+        dq.pop_front();
+        const std::vector<int> c = cfg[sc];
+        for (int i = 0; i < rank; i++)
+            for (int j = i + 1; j < rank; j++)
+                for (int k = j + 1; k <= rank; k++) {
+                    std::vector<int> n;
+                    n.insert(n.end(), c.begin(), c.begin() + i);
+                    n.insert(n.end(), c.begin() + j, c.begin() + k);
+                    n.insert(n.end(), c.begin() + i, c.begin() + j);
+                    n.insert(n.end(), c.begin() + k, c.end());
+                    int64_t gb = 1, gc = 1;
+                    for (int p = i; p < j; p++) gb *= dims[c[p]];
+                    for (int p = j; p < k; p++) gc *= dims[c[p]];
+                    const int w = (gb == 1 || gc == 1) ? 0 : 1;
+                    const int sn = encode(n.data());
+                    if (dist[sn] >= 0 && dist[sn] <= dist[sc] + w) continue;
+                    dist[sn] = dist[sc] + w;
+                    prev[sn] = sc;
+                    mi[sn] = i;
+                    mj[sn] = j;
+                    mk[sn] = k;
+                    cfg[sn] = n;
+                    if (w == 0) dq.push_front(sn);
+                    else dq.push_back(sn);
+                }
     }
-    int path[3], len = 0;
-    for (int c = code(perm); prev[c] != -1; c = prev[c]) path[len++] = move[c];
+    const int target = encode(perm);
+    if (dist[target] < 0) return IPT_ERR_INVALID_ARG;
+    std::vector<int> path;
+    for (int sc = target; sc != s0; sc = prev[sc]) path.push_back(sc);
     DeviceCtx* ctx;
     st = get_ctx(&ctx);
     if (st != IPT_OK) return st;
-    for (int k = len - 1; k >= 0; k--) {
-        const int64_t a = dims[cur[0]], b = dims[cur[1]], c = dims[cur[2]];
-        switch (path[k]) {
-        case 0: st = ipt_transpose_213(data, elem_bytes, a, b, c, stream); break;
-        case 1:
-            if (a * elem_bytes == 4 || (a * elem_bytes == 8 && (uintptr_t)data % 8 == 0))
-                st = ipt_transpose_213(data, (int)(a * elem_bytes), b, c, 1, stream);
-            else
-                st = run_chunk_transpose(ctx, data, elem_bytes, a, c, b, (hipStream_t)stream);
-            break;
-        case 2: st = ipt_transpose_213(data, elem_bytes, a, b * c, 1, stream); break;
-        default: st = ipt_transpose_213(data, elem_bytes, a * b, c, 1, stream); break;
-        }
+    std::vector<int> cur = start;
+    for (auto it = path.rbegin(); it != path.rend(); ++it) {
+        st = permute_move(ctx, data, elem_bytes, dims, rank, cur.data(), mi[*it], mj[*it], mk[*it],
+                          stream);
         if (st != IPT_OK) return st;
-        int n[3];
-        apply(path[k], cur, n);
-        std::copy(n, n + 3, cur);
+        cur = cfg[*it];
     }
     return IPT_OK;
+}
+
+ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
+                        const int* perm, void* stream) {
+    const int64_t dims[3] = {d1, d2, d3};
+    return ipt_permute(data, elem_bytes, 3, dims, perm, stream);
 }
 
 int ipt_variant_feasible(int variant, int elem_bytes, int64_t d1, int64_t d2, int64_t d3) {

@@ -442,12 +442,12 @@ static ipt_status chunk_step(DeviceCtx* ctx, void* data, const Shape& s, const B
     return stt != IPT_OK ? stt : r;
 }
 
-// Transpose of a rows x cols matrix of `inner`-element chunks (the data is
-// [rows][cols][inner], slowest first; the result [cols][rows][inner]): the
-// chunk step alone, on one slice of rows * cols chunks.  Chunks up to the
+// Transpose of `slices` rows x cols matrices of `inner`-element chunks (the
+// data is [slices][rows][cols][inner], slowest first; the result
+// [slices][cols][rows][inner]): the chunk step alone.  Chunks up to the
 // staging size (kBcStageBytes); UNSUPPORTED beyond, or past kBcMaxChunks.
 ipt_status run_chunk_transpose(DeviceCtx* ctx, void* data, int esz, int64_t inner, int64_t rows,
-                               int64_t cols, hipStream_t st) {
+                               int64_t cols, int64_t slices, hipStream_t st) {
     if (rows == 1 || cols == 1) return IPT_OK;
     const int64_t chunk = inner * esz;
     if (chunk > kBcStageBytes || rows * cols >= kBcMaxChunks) return IPT_ERR_UNSUPPORTED;
@@ -462,7 +462,7 @@ ipt_status run_chunk_transpose(DeviceCtx* ctx, void* data, int esz, int64_t inne
     p.cv = chunk / p.vb;
     p = block_cycle_for(p, data);
     if (!p.ok) return IPT_ERR_UNSUPPORTED;
-    return chunk_step(ctx, data, make_shape(esz, inner, rows * cols, 1), p, st);
+    return chunk_step(ctx, data, make_shape(esz, inner, rows * cols, slices), p, st);
 }
 
 ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,

@@ -152,9 +152,10 @@ struct BlockCyclePlan {
 BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s);
 // The plan for data at `data`: vectors narrowed to the data's alignment.
 BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data);
-// [rows][cols][inner] -> [cols][rows][inner] by the chunk step alone.
+// [slices][rows][cols][inner] -> [slices][cols][rows][inner] by the chunk
+// step alone.
 ipt_status run_chunk_transpose(DeviceCtx* ctx, void* data, int esz, int64_t inner, int64_t rows,
-                               int64_t cols, hipStream_t st);
+                               int64_t cols, int64_t slices, hipStream_t st);
 ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,
                            hipStream_t st);
 

@@ -93,6 +93,15 @@ ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m,
 ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
                         const int* perm, void* stream);
 
+/* Any permutation of a tensor of rank 1..6 in place (same convention: dims
+ * fastest first, output dim j is source dim perm[j]).  Each pass exchanges
+ * two adjacent groups of dims, the dims before them kept innermost: a 213
+ * transpose (of elements or of 4-/8-byte super-elements) or a transpose of
+ * matrices of inner chunks (chunks up to 64 KB, IPT_ERR_UNSUPPORTED
+ * beyond); the fewest passes are chosen (dims of size 1 cost nothing). */
+ipt_status ipt_permute(void* data, int elem_bytes, int rank, const int64_t* dims, const int* perm,
+                       void* stream);
+
 /* ---- variants (used by the autotuner and the selector) ---------------- */
 
 typedef enum {

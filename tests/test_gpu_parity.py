@@ -274,11 +274,38 @@ def test_permute3_matches_oracle(ipt, oracle, esz):
     ipt.device_check()
 
 
+@pytest.mark.parametrize("esz", [4, 8])
+def test_permute_rank_n_matches_oracle(ipt, oracle, esz):
+    """Random permutations of ranks 1-6 (dims of 1-9, some of size 1) against
+    the oracle's rank-N seq_tt."""
+    rng = np.random.default_rng(77 + esz)
+    for k in range(40):
+        rank = int(rng.integers(1, 7))
+        dims = [int(x) for x in rng.integers(1, 10, size=rank)]
+        if rank >= 5:
+            dims = [max(1, d // 2) for d in dims]
+        perm = [int(x) for x in rng.permutation(rank)]
+        n = int(np.prod(dims))
+        off = k % 2
+        print(f"[case] permute dims={dims} perm={perm} esz={esz} offset={off}", flush=True)
+        a = oracle.random_bits(n + 1, esz, 3500 + k)
+        t = to_dev(a)
+        ipt.permute(t[off:off + n], dims, perm)
+        torch.cuda.synchronize()
+        got = to_host(t, esz)
+        exp = a.copy()
+        exp[off:off + n] = oracle.seq_tt(a[off:off + n], dims, perm)
+        assert np.array_equal(got, exp), (dims, perm, esz, off)
+    ipt.device_check()
+
+
 def test_permute3_rejects_bad_perm(ipt):
     t = torch.zeros(24, dtype=torch.int32, device="cuda")
     for perm in [(0, 0, 1), (0, 1, 3), (-1, 1, 2)]:
         with pytest.raises(ipt.TransposeError):
             ipt.permute3(t, (2, 3, 4), perm)
+    with pytest.raises(ipt.TransposeError):
+        ipt.permute(t, (2, 3, 2, 2, 1, 1, 1), (0, 1, 2, 3, 4, 5, 6))  # rank 7
 
 
 def test_reference_entry_points(ipt, oracle):
