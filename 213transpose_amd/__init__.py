@@ -11,7 +11,10 @@ ctypes and mirrors the reference's interface (include/transpose.h):
 
 `data` is a contiguous CUDA (HIP) tensor of 4- or 8-byte elements holding d3
 slices of d2 rows x d1 columns; every slice is replaced by its transpose.  Work
-is enqueued on torch's current stream.  There is no CPU fallback: if the
+is enqueued on torch's current stream.  A pinned host tensor is transposed in
+place by the GPU over the host link (synchronise before reading it); managed
+memory, reached through the C ABI, is prefetched to the device first, as the
+reference does (src/transpose.cu:100-104).  There is no CPU fallback: if the
 library is missing or a call fails, a TransposeError is raised.
 
 The package directory name starts with a digit, so import it with
@@ -162,6 +165,8 @@ def lib():
     L.ipt_selector_active.argtypes = []
     L.ipt_selector_load.restype = _int
     L.ipt_selector_load.argtypes = [_vp]
+    L.ipt_memory_kind.restype = _int
+    L.ipt_memory_kind.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.ipt_version.restype = ctypes.c_char_p
     L.ipt_version.argtypes = []
     _lib = L
@@ -181,8 +186,8 @@ def _stream_ptr(stream):
 
 
 def _args(data, d1, d2, d3):
-    if not data.is_cuda:
-        raise TransposeError(1, "data must be a GPU tensor (no CPU path)")
+    if not (data.is_cuda or data.is_pinned()):
+        raise TransposeError(1, "data must be a GPU tensor or pinned host memory (no CPU path)")
     if not data.is_contiguous():
         raise TransposeError(1, "data must be contiguous")
     esz = data.element_size()
@@ -191,6 +196,20 @@ def _args(data, d1, d2, d3):
     if d1 * d2 * d3 != data.numel():
         raise TransposeError(1, f"d1*d2*d3 = {d1 * d2 * d3} != numel {data.numel()}")
     return ctypes.c_void_p(data.data_ptr()), esz
+
+
+# ipt_memory_kind_t: where a pointer lives (hipMemoryType's values).
+MEM_PAGEABLE, MEM_HOST, MEM_DEVICE, MEM_MANAGED = 0, 1, 2, 3
+MEM_NAMES = {MEM_PAGEABLE: "pageable", MEM_HOST: "host", MEM_DEVICE: "device", MEM_MANAGED: "managed"}
+
+
+def memory_kind(ptr):
+    """ipt_memory_kind: (kind, owning device or -1) of a pointer or tensor."""
+    if hasattr(ptr, "data_ptr"):
+        ptr = ptr.data_ptr()
+    dev = ctypes.c_int(-1)
+    k = lib().ipt_memory_kind(ctypes.c_void_p(ptr), ctypes.byref(dev))
+    return k, dev.value
 
 
 def transpose(data, d1, d2, d3=1, stream=None):

@@ -2,9 +2,10 @@
 //
 // The dispatcher mirrors inplace::transpose<T> (src/transpose.cu:99-124):
 // pick a decomposition for the slice shape, then run its passes as
-// stream-ordered kernels.  Unlike the reference it never prefetches managed
-// memory, never prints, never allocates per call (scratch is cached per
-// device) and reports every failure as an ipt_status.
+// stream-ordered kernels.  Like the reference it prefetches managed memory
+// to the current device first; unlike it, it never prints, never allocates
+// per call (scratch is cached per device), refuses memory the device cannot
+// reach instead of faulting, and reports every failure as an ipt_status.
 #include "../../include/inplace_transpose.h"
 #include "ipt_internal.h"
 #include "ipt_select.h"
@@ -63,6 +64,12 @@ ipt_status get_ctx(DeviceCtx** out) {
         nxcd = 1;
     }
     c->nxcd = nxcd;
+    int pageable = 0;
+    if (hipDeviceGetAttribute(&pageable, hipDeviceAttributePageableMemoryAccess, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        pageable = 0;
+    }
+    c->pageable = pageable != 0;
     c->sync.zero_at_rest = true;
     c->lds_max = (int64_t)prop.sharedMemPerBlock;
     snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
@@ -166,6 +173,66 @@ ipt_status validate(const void* data, int esz, int64_t d1, int64_t d2, int64_t d
     return IPT_OK;
 }
 
+// Memory kind of a pointer: hipMemoryType's value (unified memory counts as
+// managed); pageable host memory when HIP does not know the pointer, whether
+// it says so with an error or with hipMemoryTypeUnregistered.
+static int memory_kind(const void* data, int* device) {
+    hipPointerAttribute_t at{};
+    if (device) *device = -1;
+    if (hipPointerGetAttributes(&at, data) != hipSuccess) {
+        (void)hipGetLastError();
+        return IPT_MEM_PAGEABLE;
+    }
+    if (at.isManaged || at.type == hipMemoryTypeManaged || at.type == hipMemoryTypeUnified)
+        return IPT_MEM_MANAGED;
+    if (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeArray) {
+        if (device) *device = at.device;
+        return IPT_MEM_DEVICE;
+    }
+    if (at.type == hipMemoryTypeHost) return IPT_MEM_HOST;
+    return IPT_MEM_PAGEABLE;
+}
+
+// The reference prefetches managed memory to the current device before its
+// passes (src/transpose.cu:100-104) and exits on any other pointer.  Here
+// every pointer the device reaches is accepted; one it does not reach is
+// refused before anything is enqueued, since a kernel touching it would
+// fault.  A prefetch that the system cannot honour leaves the data where it
+// is, still reachable, and is not an error.
+ipt_status prepare_memory(const DeviceCtx* ctx, const void* data, size_t bytes, hipStream_t s,
+                          bool prefetch, int* kind) {
+    int owner = -1;
+    const int k = memory_kind(data, &owner);
+    if (kind) *kind = k;
+    switch (k) {
+    case IPT_MEM_DEVICE:
+        if (owner != ctx->dev) {
+            snprintf(g_last_error, sizeof(g_last_error),
+                     "data is device memory of device %d, the current device is %d", owner, ctx->dev);
+            return IPT_ERR_INVALID_ARG;
+        }
+        return IPT_OK;
+    case IPT_MEM_HOST:
+        return IPT_OK;
+    case IPT_MEM_MANAGED: {
+        if (!prefetch || bytes == 0) return IPT_OK;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+            (void)hipGetLastError();
+            return IPT_OK;
+        }
+        if (cs != hipStreamCaptureStatusNone) return IPT_OK;
+        if (hipMemPrefetchAsync(data, bytes, ctx->dev, s) != hipSuccess) (void)hipGetLastError();
+        return IPT_OK;
+    }
+    default:
+        if (ctx->pageable) return IPT_OK;
+        snprintf(g_last_error, sizeof(g_last_error),
+                 "data is pageable host memory, which device %d cannot access", ctx->dev);
+        return IPT_ERR_INVALID_ARG;
+    }
+}
+
 // IPT_VARIANT_AUTO: the learned selector's candidate when a selector is
 // loaded and no tune is forced, else the heuristic's variant.  When the
 // block-cycle path is chosen but cannot run (unaligned data, more saved
@@ -217,6 +284,12 @@ const char* ipt_status_string(ipt_status s) {
 
 const char* ipt_last_error(void) { return g_last_error; }
 
+int ipt_memory_kind(const void* data, int* device) {
+    if (device) *device = -1;
+    if (!data) return IPT_MEM_PAGEABLE;
+    return memory_kind(data, device);
+}
+
 ipt_status ipt_transpose_variant(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
                                  int variant, const ipt_tune* tune, void* stream) {
     ipt_status st = validate(data, elem_bytes, d1, d2, d3);
@@ -224,6 +297,8 @@ ipt_status ipt_transpose_variant(void* data, int elem_bytes, int64_t d1, int64_t
     if (variant < 0 || variant >= IPT_VARIANT_COUNT) return IPT_ERR_INVALID_ARG;
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    const size_t bytes = (size_t)d1 * (size_t)d2 * (size_t)d3 * (size_t)elem_bytes;
+    if ((st = prepare_memory(ctx, data, bytes, (hipStream_t)stream, true)) != IPT_OK) return st;
     Shape s = make_shape(elem_bytes, d1, d2, d3);
     if (variant == IPT_VARIANT_AUTO) return run_auto(ctx, data, s, tune, (hipStream_t)stream);
     return run_variant(ctx, data, s, variant, resolve_tune(ctx, tune), (hipStream_t)stream);
@@ -344,6 +419,8 @@ ipt_status ipt_permute(void* data, int elem_bytes, int rank, const int64_t* dims
     DeviceCtx* ctx;
     st = get_ctx(&ctx);
     if (st != IPT_OK) return st;
+    st = prepare_memory(ctx, data, (size_t)total * (size_t)elem_bytes, (hipStream_t)stream, true);
+    if (st != IPT_OK) return st;
     std::vector<int> cur = start;
     for (auto it = path.rbegin(); it != path.rend(); ++it) {
         st = permute_move(ctx, data, elem_bytes, dims, rank, cur.data(), mi[*it], mj[*it], mk[*it],
@@ -405,6 +482,9 @@ ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, 
     if (map_id < 0 || map_id > MAP_R2C_SHUFFLE) return IPT_ERR_INVALID_ARG;
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    st = prepare_memory(ctx, data, (size_t)R * (size_t)C * (size_t)d3 * (size_t)elem_bytes,
+                        (hipStream_t)stream, true);
+    if (st != IPT_OK) return st;
     return run_pass(ctx, data, elem_bytes, R, C, d3, map_id, resolve_tune(ctx, tune),
                     (hipStream_t)stream);
 }
@@ -414,6 +494,8 @@ ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream
     DeviceCtx* ctx;
     ipt_status st = get_ctx(&ctx);
     if (st != IPT_OK) return st;
+    if ((st = prepare_memory(ctx, dst, bytes, (hipStream_t)stream, true)) != IPT_OK) return st;
+    if ((st = prepare_memory(ctx, src, bytes, (hipStream_t)stream, true)) != IPT_OK) return st;
     return run_copy(ctx, dst, src, bytes, (hipStream_t)stream);
 }
 

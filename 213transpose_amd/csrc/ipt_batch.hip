@@ -107,6 +107,8 @@ struct ipt_batch_s {
     std::vector<Chunk> pre, post;  // before / after k_batch
     std::vector<ChunkMoves> moves;  // owned by the batch
     void* d_head = nullptr;         // saved heads, shared by the chunk steps in turn
+    // Managed instances (data, bytes), prefetched to the device before each run.
+    std::vector<std::pair<void*, size_t>> managed;
     Tune tune{};
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
@@ -228,9 +230,19 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     DeviceCtx* ctx;
     ipt_status st = get_ctx(&ctx);
     if (st != IPT_OK) return st;
+    std::vector<std::pair<void*, size_t>> managed;
+    for (int64_t i = 0; i < count; i++) {
+        const size_t bytes = (size_t)descs[i].d1 * (size_t)descs[i].d2 * (size_t)descs[i].d3 *
+                             (size_t)descs[i].elem_bytes;
+        int kind = 0;
+        if ((st = prepare_memory(ctx, descs[i].data, bytes, nullptr, false, &kind)) != IPT_OK)
+            return st;
+        if (kind == IPT_MEM_MANAGED) managed.emplace_back(descs[i].data, bytes);
+    }
     ipt_batch* b = new (std::nothrow) ipt_batch;
     if (!b) return IPT_ERR_OUT_OF_MEMORY;
     b->ctx = ctx;
+    b->managed = std::move(managed);
     b->tune = resolve_tune(ctx, tune);
     // One workgroup of kBatchBlock threads per CU at this LDS size (1 WG/CU
     // is the residency the group barriers rely on).
@@ -488,8 +500,20 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     return IPT_OK;
 }
 
+// Managed instances are prefetched to the device on the caller's stream
+// before each run (outside the graph), as the per-call entry points do.
+static ipt_status prefetch_managed(ipt_batch* b, hipStream_t st) {
+    for (const auto& m : b->managed) {
+        const ipt_status r = prepare_memory(b->ctx, m.first, m.second, st, true);
+        if (r != IPT_OK) return r;
+    }
+    return IPT_OK;
+}
+
 ipt_status ipt_batch_run_eager(ipt_batch* b, void* stream) {
     if (!b) return IPT_ERR_INVALID_ARG;
+    const ipt_status r = prefetch_managed(b, (hipStream_t)stream);
+    if (r != IPT_OK) return r;
     return enqueue(b, (hipStream_t)stream);
 }
 
@@ -511,6 +535,8 @@ ipt_status ipt_batch_run(ipt_batch* b, void* stream) {
             return IPT_ERR_HIP;
         }
     }
+    const ipt_status r = prefetch_managed(b, (hipStream_t)stream);
+    if (r != IPT_OK) return r;
     hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
     if (e != hipSuccess) {
         set_error("hipGraphLaunch", e);

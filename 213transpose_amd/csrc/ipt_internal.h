@@ -36,6 +36,7 @@ struct DeviceCtx {
     std::mutex mu;  // guards the arenas and serialises enqueues that use them
     Arena scratch;  // fallback-tier staging
     Arena sync;     // group-barrier counters (zero at rest)
+    bool pageable = false;  // kernels may touch pageable host memory (HMM)
 };
 
 // One 213 problem: d3 slices of d2 rows x d1 columns, esz-byte elements.
@@ -83,6 +84,11 @@ ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr);
 ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev);
 ipt_status arena_release(Arena& a, hipStream_t s);
 ipt_status validate(const void* data, int esz, int64_t d1, int64_t d2, int64_t d3);
+// Refuse memory the current device cannot reach and prefetch managed memory
+// (when `prefetch`) to the device on stream s; *kind (when not null)
+// receives the ipt_memory_kind_t of `data`.
+ipt_status prepare_memory(const DeviceCtx* ctx, const void* data, size_t bytes, hipStream_t s,
+                          bool prefetch, int* kind = nullptr);
 Shape make_shape(int esz, int64_t d1, int64_t d2, int64_t d3);
 
 Tune resolve_tune(const DeviceCtx* ctx, const ipt_tune* t);

@@ -102,7 +102,8 @@ static bool candidate_feasible(const DeviceCtx* ctx, const Shape& s, int k) {
 
 static std::mutex g_sel_mu;
 static std::vector<float> g_sel_params;  // empty = no selector
-static std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> g_sel_cache;
+// Selector choices per (device, shape): feasibility depends on the device.
+static std::map<std::tuple<int, int, int64_t, int64_t, int64_t>, int> g_sel_cache;
 static bool g_sel_default_tried = false;
 // Measured choices: exact shape -> fastest candidate, filled by ipt_autotune
 // (and ipt_tuning_load).  Consulted before the selector.
@@ -154,7 +155,7 @@ int select_candidate(const DeviceCtx* ctx, const Shape& s) {
         if (tuned != g_tuned.end() && candidate_feasible(ctx, s, tuned->second)) return tuned->second;
         load_default_weights_locked();
         if (!g_sel_params.empty() && in_domain) {
-            auto key = std::make_tuple(s.esz, s.d1, s.d2, s.d3);
+            auto key = std::make_tuple(ctx ? ctx->dev : -1, s.esz, s.d1, s.d2, s.d3);
             auto it = g_sel_cache.find(key);
             if (it != g_sel_cache.end()) return it->second;
             float x[kFeat], y[kOut];
@@ -316,6 +317,8 @@ ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
     if (!times_us || reps < 1) return IPT_ERR_INVALID_ARG;
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    const size_t bytes = (size_t)d1 * (size_t)d2 * (size_t)d3 * (size_t)elem_bytes;
+    if ((st = prepare_memory(ctx, data, bytes, nullptr, true)) != IPT_OK) return st;
     const Shape s = make_shape(elem_bytes, d1, d2, d3);
     hipStream_t stream = nullptr;
     hipEvent_t a = nullptr, b = nullptr;

@@ -14,7 +14,16 @@
  * long long, float and double instantiations, src/transpose.cu:126-129).
  *
  * Conventions (all entry points):
- *   - `data` is device memory owned by the caller (hipMalloc'd or managed).
+ *   - `data` is memory the current device can reach, owned by the caller:
+ *     device memory of the current device (hipMalloc), managed memory
+ *     (hipMallocManaged) or pinned / registered host memory.  Managed memory
+ *     is first prefetched to the current device on `stream`, as the reference
+ *     does (src/transpose.cu:100-104), except while `stream` is being
+ *     captured; where the system keeps managed memory on the host the
+ *     prefetch moves nothing and, like pinned host memory, the data is
+ *     transposed in place over the host link.  Pageable host memory (malloc,
+ *     stack) and device memory of another device are refused with
+ *     IPT_ERR_INVALID_ARG before anything is enqueued (ipt_memory_kind).
  *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
  *     Work is stream-ordered; the calls return once it is enqueued.
  *   - Arguments are validated before any HIP call.  Nothing prints, throws or
@@ -49,6 +58,23 @@ const char* ipt_status_string(ipt_status s);
 /* Detail of the last IPT_ERR_HIP / IPT_ERR_OUT_OF_MEMORY on the calling
  * thread (thread-local storage; empty when none). */
 const char* ipt_last_error(void);
+
+/* Where a pointer lives, as the entry points classify it
+ * (hipPointerGetAttributes; the values are hipMemoryType's). */
+typedef enum {
+    IPT_MEM_PAGEABLE = 0, /* host memory unknown to HIP: refused unless the device
+                             can access pageable memory */
+    IPT_MEM_HOST = 1,     /* pinned / registered host memory: used in place */
+    IPT_MEM_DEVICE = 2,   /* device memory (hipMalloc) */
+    IPT_MEM_MANAGED = 3   /* managed memory: prefetched to the current device */
+} ipt_memory_kind_t;
+
+/* Kind of the memory at `data` (an ipt_memory_kind_t); *device, when not
+ * NULL, receives the owning device of device memory, else -1.  Enqueues
+ * nothing.  Replaces the pointer requirement implied by the reference's
+ * cudaMemPrefetchAsync call (src/transpose.cu:104), which exits on anything
+ * but managed memory. */
+int ipt_memory_kind(const void* data, int* device);
 
 /* ---- transpose entry points ------------------------------------------- */
 

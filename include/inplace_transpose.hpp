@@ -11,7 +11,9 @@
 //                                                               include/openmp.h:7-8
 //
 // T is any 4- or 8-byte type (the reference instantiates int, long long, float,
-// double).  `data` is device memory (hipMalloc'd or managed).  Calls are
+// double).  `data` is device memory of the current device, managed memory
+// (prefetched to the device first, as the reference does) or pinned host
+// memory; pageable host memory is refused.  Calls are
 // enqueued on the null stream, like the reference; the overloads taking a
 // hipStream_t-compatible `void* stream` enqueue there instead.
 //
