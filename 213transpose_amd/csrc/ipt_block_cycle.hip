@@ -80,28 +80,39 @@ k_chunk_save(const V* __restrict__ data, V* __restrict__ head, const uint32_t* _
     }
 }
 
-// One workgroup per (slice, item): the item's moves are read into LDS, its
-// source chunks staged in LDS (the saved head from `head`, the rest from the
-// slice), then each staged chunk written to its destination.  Staging lanes
-// past the end repeat the last vector, so the loads carry no branches and a
-// batch of U of them is in flight at once.
+// One workgroup per (slice, item, part): the item's moves are read into LDS,
+// part `part` (vectors [v0, v0 + pv)) of its source chunks staged in LDS (the
+// saved head's from `head`, the rest from the slice), then each staged part
+// written to its destination.  Every part of a chunk follows the same cycle,
+// so the parts of one item are independent work; splitting large chunks
+// lets a segment hold up to kBcStageBytes / part chunks instead of
+// kBcStageBytes / chunk, and fewer segments means fewer saved heads.
+// Staging lanes past the end repeat the last vector, so the loads carry no
+// branches and a batch of U of them is in flight at once.  dpv divides by
+// the part length, dpv_last by the last part's (cv - (nparts - 1) * pv).
 template <typename V>
 __global__ void __launch_bounds__(kChunkBlock)
 k_chunk_cycle(V* __restrict__ data, const V* __restrict__ head, const uint2* __restrict__ items,
               const uint32_t* __restrict__ msrc, const uint32_t* __restrict__ mdst, uint32_t nitems,
-              uint32_t nheads, uint64_t nslices, uint64_t slice_vecs, uint32_t cv, Div32 dcv) {
+              uint32_t nheads, uint64_t nslices, uint64_t slice_vecs, uint32_t cv, uint32_t nparts,
+              Div32 dpv, Div32 dpv_last) {
     IPT_DYN_LDS(V, stage);
     __shared__ uint32_t src[kBcMaxSeg], dst[kBcMaxSeg];
     // 32 KB of loads in flight per workgroup whatever the vector width (with
     // 8 loads of 4 bytes per thread the chunk step of an f32 1 GB slice ran
     // at 3.1 TB/s)
     constexpr uint32_t U = 8 * 16 / sizeof(V);
-    const uint64_t total_items = nslices * nitems;
+    const uint64_t total_items = nslices * nitems * nparts;
     for (uint64_t it = blockIdx.x; it < total_items; it += gridDim.x) {
-        const uint64_t sl = it / nitems;
-        const uint2 item = items[it - sl * nitems];
+        const uint64_t seg = it / nparts;
+        const uint32_t part = (uint32_t)(it - seg * nparts);
+        const uint64_t sl = seg / nitems;
+        const uint2 item = items[seg - sl * nitems];
         const uint32_t nm = item.y;
-        V* base = data + sl * slice_vecs;
+        const Div32 dp = part + 1 == nparts ? dpv_last : dpv;
+        const uint32_t pv = dp.d;
+        const uint32_t v0 = part * dpv.d;
+        V* base = data + sl * slice_vecs + v0;
         for (uint32_t j = threadIdx.x; j < nm; j += kChunkBlock) {
             src[j] = msrc[item.x + j];
             dst[j] = mdst[item.x + j];
@@ -109,28 +120,28 @@ k_chunk_cycle(V* __restrict__ data, const V* __restrict__ head, const uint2* __r
         __syncthreads();
         uint32_t first = 0;
         if (src[0] & kHeadFlag) {  // a cut cycle's segment: its head was saved
-            const V* h = head + (sl * nheads + (src[0] & ~kHeadFlag)) * cv;
-            for (uint32_t v = threadIdx.x; v < cv; v += kChunkBlock) stage[v] = h[v];
+            const V* h = head + (sl * nheads + (src[0] & ~kHeadFlag)) * cv + v0;
+            for (uint32_t v = threadIdx.x; v < pv; v += kChunkBlock) stage[v] = h[v];
             first = 1;
         }
-        const uint32_t total = (nm - first) * cv;
+        const uint32_t total = (nm - first) * pv;
         for (uint32_t e0 = 0; e0 < total; e0 += U * kChunkBlock) {
             V r[U];
             uint32_t at[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
                 const uint32_t e = min(e0 + u * kChunkBlock + threadIdx.x, total - 1);
-                const uint32_t j = dcv.div(e), v = e - j * cv;
-                at[u] = first * cv + e;
+                const uint32_t j = dp.div(e), v = e - j * pv;
+                at[u] = first * pv + e;
                 r[u] = base[(uint64_t)src[first + j] * cv + v];
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) stage[at[u]] = r[u];
         }
         __syncthreads();
-        const uint32_t wtotal = nm * cv;
+        const uint32_t wtotal = nm * pv;
         for (uint32_t e = threadIdx.x; e < wtotal; e += kChunkBlock) {
-            const uint32_t j = dcv.div(e), v = e - j * cv;
+            const uint32_t j = dp.div(e), v = e - j * pv;
             base[(uint64_t)dst[j] * cv + v] = stage[e];
         }
         __syncthreads();  // stage, src and dst are reused by the next item
@@ -150,18 +161,22 @@ k_chunk_cycle(V* __restrict__ data, const V* __restrict__ head, const uint2* __r
 //   block step  1 / efficiency of its kernel relative to a copy (measured:
 //               square swap ~1, whole-slice 16-byte form ~0.9, cooperative
 //               kernel ~0.95 with groups on one XCD, ~0.8 across XCDs, ~0.6
-//               with the largest register-tile class, whole-slice scalar
+//               with the largest register-tile class, ~0.4 when a block
+//               side is narrower than the wave tile, whole-slice scalar
 //               form ~0.5).
-// The cheapest wins; ties go to the larger chunk.
+// The cheapest wins; ties go to the smaller chunk (smaller blocks).
 static int block_kernel(const DeviceCtx* ctx, const Shape& blk, double* eff) {
     const int64_t bytes = blk.d1 * blk.d2 * blk.esz;
     if (blk.d1 == blk.d2) { *eff = 1.0; return IPT_VARIANT_SQUARE_SWAP; }
-    const GcsGeom gg = gcs_geom(ctx, blk, lds_max_of(ctx), true);
+    const GcsGeom gg = gcs_geom(ctx, blk, lds_max_of(ctx), true, 2, true);
     const bool slice = bytes <= kBcLdsBlockBytes && variant_feasible(ctx, blk, IPT_VARIANT_SLICE_LDS);
     if (slice && (bytes % 16) == 0) { *eff = 0.9; return IPT_VARIANT_SLICE_LDS; }
     if (gg.ok) {
         const int nx = std::max(ctx ? ctx->nxcd : 1, 1);
-        if (gcs_jr_class(blk.esz, gg.K, gg.KL, kGcsBlock / 64) == 2) *eff = 0.6;
+        const int64_t tile = blk.esz == 4 ? 32 : 16;  // wave tile side
+        const bool packed2 = gg.grid && gg.grid / gg.g >= 2;  // largest class by choice
+        if (std::min(blk.d1, blk.d2) < tile) *eff = 0.4;  // mostly idle lanes
+        else if (gcs_jr_class(blk.esz, gg.K, gg.KL, kGcsBlock / 64) == 2 && !packed2) *eff = 0.6;
         else *eff = (int64_t)gg.g <= ncu_of(ctx) / nx ? 0.95 : 0.8;
         return IPT_VARIANT_GCS;
     }
@@ -185,26 +200,39 @@ static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, Bl
         const int v = block_kernel(ctx, blk, &eff);
         if (v == IPT_VARIANT_AUTO) continue;
         const int64_t chunk = T * esz;
-        const int64_t vb = chunk % 16 == 0 ? 16 : chunk % 8 == 0 ? 8 : 4;
-        const int64_t ell = std::max<int64_t>(1, std::min<int64_t>(kBcMaxSeg, kBcStageBytes / chunk));
+        BlockCyclePlan c{};
+        c.ok = true;
+        c.chunk_first = cols;
+        c.T = T;
+        c.nb = nb;
+        c.other = other;
+        c.rc = cols ? other : nb;
+        c.cc = cols ? nb : other;
+        c.vb = chunk % 16 == 0 ? 16 : chunk % 8 == 0 ? 8 : 4;
+        c.blk = blk;
+        c.block_variant = v;
+        chunk_parts(&c);
         const double short_run = chunk < 2048 ? 1.0 + (2048.0 - chunk) / 4096.0 : 1.0;
-        const double narrow = vb == 16 ? 1.0 : vb == 8 ? 1.15 : 1.4;
-        const double cost = (1.0 + 1.5 / ell) * short_run * narrow + 1.0 / eff;
-        if (best->ok && cost >= *best_cost - 1e-9) continue;  // ties keep the larger chunk
-        best->ok = true;
-        best->chunk_first = cols;
-        best->T = T;
-        best->nb = nb;
-        best->other = other;
-        best->rc = cols ? other : nb;
-        best->cc = cols ? nb : other;
-        best->vb = vb;
-        best->cv = chunk / vb;
-        best->ell = ell;
-        best->blk = blk;
-        best->block_variant = v;
+        const double narrow = c.vb == 16 ? 1.0 : c.vb == 8 ? 1.15 : 1.4;
+        const double cost = (1.0 + 1.5 / c.ell) * short_run * narrow + 1.0 / eff;
+        // ties go to the smaller chunk (T falls): its blocks are smaller
+        if (best->ok && cost > *best_cost + 1e-9) continue;
+        *best = c;
         *best_cost = cost;
     }
+}
+
+// Chunks of more than 2 * kBcPartBytes move in nsub parts of at least
+// kBcPartBytes; a segment stages up to kBcStageBytes of parts.  Narrowing the
+// vectors later (block_cycle_for) keeps nsub and ell: a part then holds at
+// most as many bytes as before, so the staging still fits.
+void chunk_parts(BlockCyclePlan* p) {
+    const int64_t chunk = p->T * p->blk.esz;
+    p->cv = chunk / p->vb;
+    p->nsub = std::max<int64_t>(1, chunk / kBcPartBytes);
+    if (p->nsub > p->cv) p->nsub = p->cv;
+    const int64_t part = chunk_part_vecs(*p) * p->vb;
+    p->ell = std::max<int64_t>(1, std::min<int64_t>(kBcMaxSeg, kBcStageBytes / part));
 }
 
 BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data) {
@@ -377,8 +405,10 @@ static ipt_status chunk_step_launch_v(const DeviceCtx* ctx, void* data, uint64_t
                                      uint64_t slice_vecs, const BlockCyclePlan& p,
                                      const ChunkMoves& m, void* head, uint64_t group,
                                      hipStream_t st) {
-    const size_t lds = (size_t)p.ell * p.cv * sizeof(V);
+    const uint32_t pv = (uint32_t)chunk_part_vecs(p), nparts = (uint32_t)chunk_nparts(p);
+    const size_t lds = (size_t)p.ell * pv * sizeof(V);
     const Div32 dcv = Div32::make((uint32_t)p.cv);
+    const Div32 dpv = Div32::make(pv), dpv_last = Div32::make((uint32_t)p.cv - (nparts - 1) * pv);
     for (uint64_t s0 = 0; s0 < nslices; s0 += group) {
         const uint64_t ns = std::min<uint64_t>(group, nslices - s0);
         V* d = static_cast<V*>(data) + s0 * slice_vecs;
@@ -392,10 +422,10 @@ static ipt_status chunk_step_launch_v(const DeviceCtx* ctx, void* data, uint64_t
                                (uint32_t)p.cv, dcv);
             IPT_LAUNCH_CHECK("k_chunk_save");
         }
-        const int gc = (int)std::min<uint64_t>(ns * m.nitems, (uint64_t)ncu_of(ctx) * 8);
+        const int gc = (int)std::min<uint64_t>(ns * m.nitems * nparts, (uint64_t)ncu_of(ctx) * 8);
         hipLaunchKernelGGL(k_chunk_cycle<V>, dim3(gc), dim3(kChunkBlock), lds, st, d, (const V*)head,
                            (const uint2*)m.items, (const uint32_t*)m.msrc, (const uint32_t*)m.mdst,
-                           m.nitems, m.nheads, ns, slice_vecs, (uint32_t)p.cv, dcv);
+                           m.nitems, m.nheads, ns, slice_vecs, (uint32_t)p.cv, nparts, dpv, dpv_last);
         IPT_LAUNCH_CHECK("k_chunk_cycle");
     }
     return IPT_OK;
@@ -444,13 +474,15 @@ static ipt_status chunk_step(DeviceCtx* ctx, void* data, const Shape& s, const B
 
 // Transpose of `slices` rows x cols matrices of `inner`-element chunks (the
 // data is [slices][rows][cols][inner], slowest first; the result
-// [slices][cols][rows][inner]): the chunk step alone.  Chunks up to the
-// staging size (kBcStageBytes); UNSUPPORTED beyond, or past kBcMaxChunks.
+// [slices][cols][rows][inner]): the chunk step alone.  Chunks of any size
+// (moved in parts); UNSUPPORTED past kBcMaxChunks chunks per slice, or when
+// the saved heads of one slice exceed the scratch cap.
 ipt_status run_chunk_transpose(DeviceCtx* ctx, void* data, int esz, int64_t inner, int64_t rows,
                                int64_t cols, int64_t slices, hipStream_t st) {
     if (rows == 1 || cols == 1) return IPT_OK;
     const int64_t chunk = inner * esz;
-    if (chunk > kBcStageBytes || rows * cols >= kBcMaxChunks) return IPT_ERR_UNSUPPORTED;
+    if (rows * cols >= kBcMaxChunks || chunk / 4 > (int64_t)UINT32_MAX / kBcMaxSeg)
+        return IPT_ERR_UNSUPPORTED;
     BlockCyclePlan p{};
     p.ok = true;
     p.T = inner;
@@ -458,8 +490,7 @@ ipt_status run_chunk_transpose(DeviceCtx* ctx, void* data, int esz, int64_t inne
     p.rc = rows;
     p.cc = cols;
     p.vb = chunk % 16 == 0 ? 16 : chunk % 8 == 0 ? 8 : 4;
-    p.ell = std::max<int64_t>(1, std::min<int64_t>(kBcMaxSeg, kBcStageBytes / chunk));
-    p.cv = chunk / p.vb;
+    chunk_parts(&p);
     p = block_cycle_for(p, data);
     if (!p.ok) return IPT_ERR_UNSUPPORTED;
     return chunk_step(ctx, data, make_shape(esz, inner, rows * cols, slices), p, st);

@@ -70,14 +70,24 @@ static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
 // over all CUs (larger g, less data per member); otherwise members are
 // filled to capacity.  Slices of 2 GiB or more never fit (buffer offsets are
 // 32-bit, and such a slice is far beyond what 256 members hold).  The largest
-// register-tile class is used only when a slice needs it to fit: members
+// register-tile class is used only when a slice needs it to fit, or when
+// packed groups of it share a round two or more to one of class 1: members
 // that large were slower wherever a smaller class also fits (f32 3000 x 3000
 // x 8: 3446 vs 4578 GB/s).
-GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread, int jr_cls) {
-    if (jr_cls > 1) {
-        const GcsGeom small = gcs_geom(ctx, s, lds_cap, spread, 1);
-        if (small.ok) return small;
-    }
+// Packing (several rounds of groups): g members per slice with gmin <= g
+// leave ncu mod g workgroups that start the next round's first group early,
+// and that group's other members then wait at its barrier for them to finish
+// a whole slice first.  Groups of ncu / k members (k = ncu / gmin groups per
+// round) launched on exactly k * g workgroups avoid it.  Measured on MI355X
+// (GB/s, default -> packed): f32 1669 x 5801 x 25 (9646 tiles) 2525 -> 2873,
+// and 3294 with the larger register tiles that fit two groups per round;
+// f32 2000 x 2048 x 60 4677 -> 4720; f64 1200 x 1000 x 80 4448 -> 4609.  The
+// larger register tiles pay only there: elsewhere they lost (f32 1500 x 1300
+// x 150: 13 groups per round 3553, 9 groups with class 1 3794).
+//
+// gcs_geom_cls: the geometry with register tiles of class jr_cls.
+static GcsGeom gcs_geom_cls(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread,
+                            int jr_cls, bool pack) {
     GcsGeom gg{};
     const int64_t R = s.d2, C = s.d1, esz = s.esz;
     gg.ts = esz == 4 ? 32 : 16;
@@ -93,7 +103,14 @@ GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spr
     const int64_t ncu = ncu_of(ctx);
     const int64_t gmin = (ntiles + klmax + krmax - 1) / (klmax + krmax);
     int64_t g = gmin;
-    if (spread && s.d3 * gmin <= ncu) g = std::max<int64_t>(gmin, std::min<int64_t>(ncu / s.d3, ntiles));
+    gg.grid = 0;
+    if (spread && s.d3 * gmin <= ncu) {
+        g = std::max<int64_t>(gmin, std::min<int64_t>(ncu / s.d3, ntiles));
+    } else if (pack && gmin > 1 && gmin <= ncu) {
+        const int64_t k = ncu / gmin;  // groups per round
+        g = std::min<int64_t>(ncu / k, ntiles);
+        gg.grid = (uint32_t)(k * g);
+    }
     const int64_t K = (ntiles + g - 1) / g;
     gg.ntj = (uint32_t)ntj;
     gg.ntiles = (uint32_t)ntiles;
@@ -105,6 +122,16 @@ GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spr
     // every launch of this kernel is guaranteed).
     gg.ok = g <= ncu && (uint64_t)s.d3 * (uint64_t)g < ((uint64_t)1 << 31);
     return gg;
+}
+
+GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread, int jr_cls,
+                 bool pack) {
+    if (jr_cls <= 1) return gcs_geom_cls(ctx, s, lds_cap, spread, jr_cls, pack);
+    const GcsGeom small = gcs_geom_cls(ctx, s, lds_cap, spread, 1, pack);
+    if (!small.ok) return gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack);
+    if (!pack || small.grid == 0 || small.grid / small.g != 1) return small;
+    const GcsGeom big = gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack);
+    return big.ok && big.grid && big.grid / big.g >= 2 ? big : small;
 }
 
 bool variant_feasible(const DeviceCtx* ctx, const Shape& s, int variant) {
@@ -263,7 +290,9 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
     const uint64_t resident = (uint64_t)per_cu * (uint64_t)ncu_of(ctx);
     // Never more workgroups than can be resident at once: every member of a
     // group must be running while the others wait for it.
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(slots, resident));
+    uint64_t want = std::min<uint64_t>(slots, resident);
+    if (gg.grid) want = std::min<uint64_t>(want, gg.grid);  // whole groups per round
+    const int grid = (int)std::max<uint64_t>(1, want);
     if ((uint64_t)grid < gg.g) return IPT_ERR_UNSUPPORTED;  // a group must be co-resident
     if (gg.g == 1) {  // whole slices per workgroup: no barrier state needed
         hipLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, d, (uint32_t)s.d2,
@@ -300,7 +329,7 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
 
 template <typename T>
 static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t, hipStream_t st) {
-    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true);
+    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, 2, true);
     if (!gg.ok) return IPT_ERR_UNSUPPORTED;
     switch (gcs_jr_class(sizeof(T), gg.K, gg.KL, kGcsBlock / 64)) {
     case 0: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 0)>, data, s, gg, st);

@@ -119,14 +119,17 @@ struct GcsGeom {
     uint32_t K;       // tiles per member (max)
     uint32_t KL;      // of which staged in LDS; the rest stay in registers
     size_t lds;       // bytes per workgroup
+    uint32_t grid;    // workgroups to launch; 0: as many as slots and residency allow
     bool ok;
 };
 
 // spread: when every slice fits one round, use more members per slice.
 // jr_cls: the largest register-tile class the kernel that runs it has
 // (gcs_jr_of; the batch kernel stops at kBatchJrClass).
+// pack: when slices take several rounds, size groups so that a whole number
+// of them fills the CUs, and launch exactly those workgroups.
 GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true,
-                 int jr_cls = 2);
+                 int jr_cls = 2, bool pack = false);
 int64_t slice_pitch(int64_t C);
 int64_t lds_max_of(const DeviceCtx* ctx);
 int ncu_of(const DeviceCtx* ctx);
@@ -135,8 +138,10 @@ bool decomp_needs_scratch(const Shape& s, bool c2r, const Tune& t);
 // Block-cycle path (ipt_block_cycle.hip): chunks of T elements along one
 // side moved by cycle following, blocks of other x T transposed in one sweep.
 constexpr int64_t kBcMinChunk = 512;             // bytes per moved chunk, at least
-constexpr int64_t kBcMaxChunk = 32 * 1024;       // and at most (two fit one segment's LDS)
-constexpr int64_t kBcStageBytes = 64 * 1024;     // chunks staged per segment
+constexpr int64_t kBcMaxChunk = 32 * 1024;       // and at most (larger chunks mean skinny
+                                                 // blocks, measured slower)
+constexpr int64_t kBcStageBytes = 64 * 1024;     // chunk parts staged per segment
+constexpr int64_t kBcPartBytes = 2048;           // chunks above twice this move in parts of >= it
 constexpr int64_t kBcLdsBlockBytes = 64 * 1024;  // blocks up to this go to the whole-slice kernel
 constexpr uint32_t kBcMaxSeg = 128;              // chunk moves per work item
 constexpr int64_t kBcMaxChunks = (int64_t)1 << 23;  // chunks per slice (host plan size)
@@ -151,6 +156,7 @@ struct BlockCyclePlan {
     int64_t rc = 0, cc = 0;    // chunk matrix transposed by the chunk step
     int64_t vb = 16;           // bytes per vector the chunk step moves (16, 8 or 4)
     int64_t cv = 0;            // vectors per chunk
+    int64_t nsub = 1;          // parts a chunk is moved in (each part follows the same cycles)
     int64_t ell = 0;           // chunks per segment (at most)
     Shape blk;                 // the block step as a 213 problem
     int block_variant = 0;     // its one-sweep kernel
@@ -158,6 +164,14 @@ struct BlockCyclePlan {
 BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s);
 // The plan for data at `data`: vectors narrowed to the data's alignment.
 BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data);
+// Parts and segment length of a plan whose T, blk.esz and vb are set.
+void chunk_parts(BlockCyclePlan* p);
+// Vectors per part (the last part may be shorter) and the number of parts.
+inline int64_t chunk_part_vecs(const BlockCyclePlan& p) { return (p.cv + p.nsub - 1) / p.nsub; }
+inline int64_t chunk_nparts(const BlockCyclePlan& p) {
+    const int64_t pv = chunk_part_vecs(p);
+    return (p.cv + pv - 1) / pv;
+}
 // [slices][rows][cols][inner] -> [slices][cols][rows][inner] by the chunk
 // step alone.
 ipt_status run_chunk_transpose(DeviceCtx* ctx, void* data, int esz, int64_t inner, int64_t rows,
