@@ -767,13 +767,27 @@ k_square_swap_rs(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3) 
 // before is about twice as slow when that member ran on another XCD
 // (measured on the [512][384][64] f32 permute: 24.1 -> 16.5 us per launch
 // with XCD-local groups), so slots are numbered to give each XCD one
-// contiguous range: consecutive slots, the members of one slice, then share
-// an L2.  A bijection on [0, n): the set of slots run, and the deadlock-
-// freedom argument (which is over slot order), do not depend on it.
-__device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n, uint32_t nxcd) {
+// contiguous range.  When a group fits an XCD's share (h = (n / nxcd) / g
+// >= 1 groups per XCD), the range holds whole groups: workgroup i < h * g *
+// nxcd takes slot x * h * g + i / nxcd (x = i % nxcd), so each of those
+// groups runs on one XCD and shares its L2, and the remaining workgroups
+// keep slot i (a group among them spans XCDs).  Groups of g that do not
+// divide an XCD's share then no longer straddle two XCDs (f64 [256][1000]
+// [17]: 15 members per slice, 255 workgroups; 16 of the 17 groups on one
+// XCD instead of 10).  Larger groups take consecutive ranges of XCDs (each
+// XCD one contiguous range of n / nxcd slots).  A bijection on [0, n): the
+// set of slots run, and the deadlock-freedom argument (which is over slot
+// order), do not depend on it.
+__device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n, uint32_t nxcd, uint32_t g) {
     if (nxcd <= 1) return i;
-    const uint32_t q = n / nxcd, rem = n % nxcd, x = i % nxcd;
-    return x * q + min(x, rem) + i / nxcd;
+    const uint32_t x = i % nxcd;
+    const uint32_t span = (n / nxcd / g) * g;  // slots per XCD in whole groups
+    if (span == 0) {
+        const uint32_t q = n / nxcd, rem = n % nxcd;
+        return x * q + min(x, rem) + i / nxcd;
+    }
+    if (i >= span * nxcd) return i;
+    return x * span + i / nxcd;
 }
 
 // Wave tiles a wave stages in LDS (JL) and holds in registers (JR) on the
@@ -884,7 +898,7 @@ k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g
     IPT_DYN_LDS(u32x4, lds);
     const uint64_t total = nslices * g;
     const uint64_t slice_elems = (uint64_t)R * C;
-    for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd); slot < total; slot += gridDim.x) {
+    for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd, g); slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
         gcs_body<T, BLOCK, JR, kStoreDevice>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
