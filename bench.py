@@ -250,23 +250,28 @@ def profile_main(args, torch, ipt):
 
 
 def pmc_per_dispatch(db, counter):
-    """{kernel name: mean counter value per dispatch} from a rocpd database."""
+    """{kernel name: [counter value of each dispatch, in dispatch order]}
+    from a rocpd database."""
     con = sqlite3.connect(db)
     rows = con.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
-                       "where counter_name = ? group by kernel_name, dispatch_id", (counter,))
+                       "where counter_name = ? group by kernel_name, dispatch_id "
+                       "order by dispatch_id", (counter,))
     acc = {}
     for name, _, val in rows:
         acc.setdefault(name, []).append(val)
-    return {k: sum(v) / len(v) for k, v in acc.items()}
+    return acc
 
 
 def measure_traffic(match, shape_esz, cands):
     """Bytes per launch of the kernel whose name contains `match`, from
     FETCH_SIZE and WRITE_SIZE, each scaled per byte by the copy kernel that
-    copies the same tensor (same bytes read and written per dispatch)."""
+    copies the same tensor (same bytes read and written per dispatch).  The
+    profile run copies the workload tensors in turn (profile_main), so the
+    copies of tensor i are the copy dispatches i, i + n, i + 2n, ..."""
     d1, d2, d3, esz = shape_esz
     copy_bytes = d1 * d2 * d3 * esz
     tname = "unsigned int" if esz == 4 else "unsigned long"
+    ti = WORKLOAD.index(tuple(shape_esz))
     totals = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         out = tempfile.mkdtemp(prefix="ipt_pmc_")
@@ -279,12 +284,12 @@ def measure_traffic(match, shape_esz, cands):
             return None, f"rocprofv3 --pmc {counter} failed (rc={r.returncode})"
         per = pmc_per_dispatch(dbs[0], counter)
         kern = [v for k, v in per.items() if all(part in k for part in match)]
-        # copies of both tensors appear: pick the one closest to this tensor's size
-        copies = sorted((v for k, v in per.items() if "k_copy16" in k),
-                        key=lambda v: abs(v * 1024 - copy_bytes))
-        if not copies or not kern:
+        copies = [v for k, v in per.items() if "k_copy16" in k]
+        mine = copies[0][ti::len(WORKLOAD)] if copies else []
+        if not mine or not kern:
             return None, f"{counter}: kernel not found in profile ({tname})"
-        totals[counter] = kern[0] * (copy_bytes / copies[0])
+        kern_mean = sum(kern[0]) / len(kern[0])
+        totals[counter] = kern_mean * (copy_bytes / (sum(mine) / len(mine)))
     return totals, None
 
 
