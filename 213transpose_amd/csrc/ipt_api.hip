@@ -334,8 +334,9 @@ ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m, i
 //   A = 1            the 213 transpose of (|B|, |C|, |D|);
 //   |A|*esz = 4, 8   the same on super-elements of |A| elements;
 //   otherwise        a transpose of |D| matrices of |C| x |B| chunks of |A|
-//                    elements (the block-cycle chunk step; chunks up to
-//                    64 KB, IPT_ERR_UNSUPPORTED beyond).
+//                    elements (the block-cycle chunk step, chunks of any
+//                    size moved in parts; IPT_ERR_UNSUPPORTED past its
+//                    chunk-count and head-scratch limits).
 // A move with |B| = 1 or |C| = 1 moves no data.  The layout state (cur[p] =
 // the source dim at position p, the convention of `perm`) is searched
 // breadth-first, moving no data counting zero, for the fewest passes: rank

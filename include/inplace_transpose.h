@@ -114,8 +114,9 @@ ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m,
  * reference.  One in-place pass for every permutation but the reversal
  * {2,1,0}, which takes two: 213 transposes of the dims or of merged dims,
  * and the exchange of the outer two dims with the innermost kept
- * contiguous (a transpose of a matrix of innermost-dim chunks, chunks up to
- * 64 KB; IPT_ERR_UNSUPPORTED beyond). */
+ * contiguous (a transpose of a matrix of innermost-dim chunks, of any size;
+ * IPT_ERR_UNSUPPORTED past 2^23 chunks per outer slice or when the saved
+ * cycle heads of one slice exceed the library's 256 MB scratch). */
 ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
                         const int* perm, void* stream);
 
@@ -123,8 +124,8 @@ ipt_status ipt_permute3(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
  * fastest first, output dim j is source dim perm[j]).  Each pass exchanges
  * two adjacent groups of dims, the dims before them kept innermost: a 213
  * transpose (of elements or of 4-/8-byte super-elements) or a transpose of
- * matrices of inner chunks (chunks up to 64 KB, IPT_ERR_UNSUPPORTED
- * beyond); the fewest passes are chosen (dims of size 1 cost nothing). */
+ * matrices of inner chunks (chunks of any size, limits as ipt_permute3);
+ * the fewest passes are chosen (dims of size 1 cost nothing). */
 ipt_status ipt_permute(void* data, int elem_bytes, int rank, const int64_t* dims, const int* perm,
                        void* stream);
 

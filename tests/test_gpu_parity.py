@@ -299,6 +299,24 @@ def test_permute_rank_n_matches_oracle(ipt, oracle, esz):
     ipt.device_check()
 
 
+@pytest.mark.parametrize("esz", [4, 8])
+def test_permute_large_inner_chunks(ipt, oracle, esz):
+    """Exchanges whose inner chunks are large (tens of KB, beyond one
+    segment's staging) move them in parts; odd chunk lengths narrow the
+    vectors.  Against the oracle's rank-N seq_tt."""
+    cases = [([20000, 3, 4], [0, 2, 1]), ([6001, 5, 3], [0, 2, 1]), ([3000, 7, 2, 3], [0, 3, 1, 2]),
+             ([9000, 2, 5], [0, 2, 1])]
+    for k, (dims, perm) in enumerate(cases):
+        n = int(np.prod(dims))
+        print(f"[case] permute large chunks dims={dims} perm={perm} esz={esz}", flush=True)
+        a = oracle.random_bits(n, esz, 3700 + k)
+        t = to_dev(a)
+        ipt.permute(t, dims, perm)
+        torch.cuda.synchronize()
+        assert np.array_equal(to_host(t, esz), oracle.seq_tt(a, dims, perm)), (dims, perm, esz)
+    ipt.device_check()
+
+
 def test_permute3_rejects_bad_perm(ipt):
     t = torch.zeros(24, dtype=torch.int32, device="cuda")
     for perm in [(0, 0, 1), (0, 1, 3), (-1, 1, 2)]:
