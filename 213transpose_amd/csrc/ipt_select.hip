@@ -318,11 +318,15 @@ ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
     const size_t bytes = (size_t)d1 * (size_t)d2 * (size_t)d3 * (size_t)elem_bytes;
-    if ((st = prepare_memory(ctx, data, bytes, nullptr, true)) != IPT_OK) return st;
+    if ((st = prepare_memory(ctx, data, bytes, nullptr, false)) != IPT_OK) return st;
     const Shape s = make_shape(elem_bytes, d1, d2, d3);
     hipStream_t stream = nullptr;
     hipEvent_t a = nullptr, b = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    // the caller's queued work (on any stream) finishes before the timing
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    // managed memory moves to the device on the stream the timed calls use
+    if (e == hipSuccess) st = prepare_memory(ctx, data, bytes, stream, true);
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
     int bestk = -1;

@@ -244,7 +244,7 @@ int ipt_heuristic_candidate(int elem_bytes, int64_t d1, int64_t d2, int64_t d3);
 
 /* Autotuner: time every candidate on `data` (contents are permuted; any
  * values will do), one warm-up run then `reps` timed runs each, HIP events
- * on an internal stream.  times_us[k] is the mean time in microseconds, or a
+ * on an internal stream, after a device synchronise (blocking call).  times_us[k] is the mean time in microseconds, or a
  * negative value when candidate k is infeasible for the shape.  *best is the
  * fastest candidate. */
 ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3, int reps,
