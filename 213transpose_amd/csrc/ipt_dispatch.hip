@@ -352,6 +352,30 @@ static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t
 // C=9286 (strips of one column in LDS): C2R 13.9 -> 8.7 ms with the first
 // alone.  Not under stream capture (the arena cannot grow there) and not
 // where the caller rules scratch out (batches).
+constexpr int64_t kDiagBandBytes = 128 * 1024;  // LDS band of the diagonal column maps
+
+template <typename T, typename V, typename Ix, bool C2R_POST>
+static ipt_status launch_diag(DeviceCtx* ctx, T* data, T* scr, const Plan<Ix>& p, uint32_t wshift,
+                              Div64 dstrips, uint64_t first, uint64_t n, uint32_t kb,
+                              hipStream_t st) {
+    if constexpr (sizeof(V) < sizeof(T)) {
+        return IPT_ERR_INVALID_ARG;  // 8-byte elements are always 8-byte aligned
+    } else {
+        const uint32_t W = 1u << wshift;
+        const size_t lds = (size_t)(kb + W - 1) * W * sizeof(T);
+        auto k = k_col_diag_to_scratch<T, V, Ix, C2R_POST>;
+        ipt_status stt = allow_lds(ctx, k, lds);
+        if (stt != IPT_OK) return stt;
+        const uint32_t nkb = (uint32_t)(((uint64_t)p.R + kb - 1) / kb);
+        const uint64_t units = n * nkb;
+        const int gd = (int)std::min<uint64_t>(units, (uint64_t)ncu_of(ctx) * 8);
+        hipLaunchKernelGGL(k, dim3(gd), dim3(kDiagBlock), lds, st, (const T*)data, scr, p, wshift,
+                           dstrips, first, (uint32_t)n, kb, Div32::make(nkb));
+        IPT_LAUNCH_CHECK("k_col_diag_to_scratch");
+        return IPT_OK;
+    }
+}
+
 template <typename T, typename Ix, typename Map>
 static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostPlan& h,
                          uint64_t d3, const Tune& t, hipStream_t st) {
@@ -407,14 +431,21 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
         const uint32_t total = (uint32_t)(n * per_item);
         const int grid = (int)std::min<uint64_t>(((uint64_t)total + 255) / 256, (uint64_t)ncu_of(ctx) * 16);
         if constexpr (Map::kDiag != 0) {
-            constexpr uint32_t KB = 256;  // k rows per unit
-            const uint32_t nkb = (uint32_t)(((uint64_t)R + KB - 1) / KB);
-            const uint64_t units = n * nkb;
-            const int gd = (int)std::min<uint64_t>(units, (uint64_t)ncu_of(ctx) * 8);
-            hipLaunchKernelGGL((k_col_diag_to_scratch<T, Ix, Map::kDiag == 1>), dim3(gd), dim3(256), 0,
-                               st, data, (T*)scr, p, wshift, dstrips, first, (uint32_t)n, KB,
-                               Div32::make(nkb));
-            IPT_LAUNCH_CHECK("k_col_diag_to_scratch");
+            // k rows per unit: a band of kb + W - 1 rows of W * esz = 256 bytes
+            // in at most kDiagBandBytes of LDS; row segments move as the widest
+            // vectors the row pitch and the data's alignment allow
+            const uint32_t kb = (uint32_t)std::min<int64_t>(R, kDiagBandBytes / (W * esz) - (W - 1));
+            const uintptr_t align = (uintptr_t)data | (uintptr_t)(C * esz);
+            if (align % 16 == 0)
+                stt = launch_diag<T, uint4, Ix, Map::kDiag == 1>(ctx, data, (T*)scr, p, wshift, dstrips,
+                                                                 first, n, kb, st);
+            else if (align % 8 == 0)
+                stt = launch_diag<T, uint2, Ix, Map::kDiag == 1>(ctx, data, (T*)scr, p, wshift, dstrips,
+                                                                 first, n, kb, st);
+            else
+                stt = launch_diag<T, uint32_t, Ix, Map::kDiag == 1>(ctx, data, (T*)scr, p, wshift,
+                                                                    dstrips, first, n, kb, st);
+            if (stt != IPT_OK) return stt;
         } else {
             hipLaunchKernelGGL((k_col_gather_to_scratch<T, Ix, Map>), dim3(grid), dim3(256), 0, st,
                                data, (T*)scr, p, wshift, dstrips, first, total, dper);
