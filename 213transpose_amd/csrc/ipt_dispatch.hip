@@ -421,7 +421,6 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
     const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(kScratchCap / esz, (uint64_t)INT32_MAX) / per_item);
     if (per_item > (uint64_t)INT32_MAX) return IPT_ERR_TOO_LARGE;  // one strip beyond 32-bit element math
     const Div64 dstrips = Div64::make(nstrips);
-    const Div32 dper = Div32::make((uint32_t)per_item);
     std::lock_guard<std::mutex> g(ctx->mu);
     void* scr;
     ipt_status stt = arena_acquire(ctx->scratch, (size_t)(std::min(chunk, items) * per_item * esz), st, &scr);
@@ -429,6 +428,7 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
     for (uint64_t first = 0; first < items; first += chunk) {
         const uint64_t n = std::min(chunk, items - first);
         const uint32_t total = (uint32_t)(n * per_item);
+        const Div32 drow = Div32::make((uint32_t)(n * W));  // a scratch row: the chunk's n strips
         const int grid = (int)std::min<uint64_t>(((uint64_t)total + 255) / 256, (uint64_t)ncu_of(ctx) * 16);
         if constexpr (Map::kDiag != 0) {
             // k rows per unit: a band of kb + W - 1 rows of W * esz = 256 bytes
@@ -448,11 +448,11 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
             if (stt != IPT_OK) return stt;
         } else {
             hipLaunchKernelGGL((k_col_gather_to_scratch<T, Ix, Map>), dim3(grid), dim3(256), 0, st,
-                               data, (T*)scr, p, wshift, dstrips, first, total, dper);
+                               data, (T*)scr, p, wshift, dstrips, first, total, drow);
             IPT_LAUNCH_CHECK("k_col_gather_to_scratch");
         }
         hipLaunchKernelGGL((k_col_scratch_back<T>), dim3(grid), dim3(256), 0, st, data,
-                           (const T*)scr, (uint64_t)R, (uint64_t)C, wshift, dstrips, first, total, dper);
+                           (const T*)scr, (uint64_t)R, (uint64_t)C, wshift, dstrips, first, total, drow);
         IPT_LAUNCH_CHECK("k_col_scratch_back");
     }
     return arena_release(ctx->scratch, st);

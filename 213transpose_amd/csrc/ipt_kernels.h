@@ -313,19 +313,23 @@ k_col_gather_lds(T* __restrict__ data, Plan<Ix> p, uint64_t d3, uint32_t W, uint
 // library scratch, then copy back.  Two launches, so the kernel boundary is
 // the barrier between the last read of old data and the first overwrite.
 
-// scratch[(w-first)][i][jj] = old[g(i,j)][j] for strips w in [first, first+n).
-// Element e of the chunk (total < 2^31, so 32-bit reduced division) is strip
-// e / (R*W), row (e mod R*W) >> wshift, lane jj = e & (W - 1): consecutive
-// lanes walk one row segment of W columns, so a map whose source row is the
-// same across the strip (a rotation by a per-run constant) reads W*esz
-// contiguous bytes.  U elements per thread in flight; lanes past the end
-// repeat the last element, and columns past C (a partial last strip) read
-// column C - 1 into scratch slots the copy back never reads.
+// Scratch of a chunk of n strips w in [first, first + n) is row-major across
+// the chunk: scratch[i][w - first][jj] holds new[i][j] of strip w (element
+// e = (i * n + w - first) * W + jj), so a scratch row is a run of n * W
+// columns and the copy back writes each data row's run contiguously (with
+// 256-byte strips strided by the row pitch the copy back ran at 3.3 TB/s).
+// k_col_gather_to_scratch: scratch[i][w - first][jj] = old[g(i,j)][j].
+// Element e of the chunk (total < 2^31, so 32-bit reduced division by the
+// scratch row n * W): consecutive lanes walk one row's run of columns, so a
+// map whose source row is constant across runs (a rotation by a per-run
+// constant) reads contiguous bytes.  U elements per thread in flight; lanes
+// past the end repeat the last element, and columns past C (a partial last
+// strip) read column C - 1 into scratch slots the copy back never reads.
 template <typename T, typename Ix, typename Map>
 __global__ void __launch_bounds__(256)
 k_col_gather_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<Ix> p,
                         uint32_t wshift, Div64 dstrips, uint64_t first, uint32_t total,
-                        Div32 dper) {
+                        Div32 drow) {
     constexpr uint32_t U = 8;
     const uint64_t C = p.C;
     const uint32_t W = 1u << wshift;
@@ -336,8 +340,8 @@ k_col_gather_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Pla
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t e = min(e0 + u * stride, total - 1);
-            const uint32_t wl = dper.div(e), rem = e - wl * dper.d;
-            const uint32_t i = rem >> wshift, jj = rem & (W - 1);
+            const uint32_t i = drow.div(e), rem = e - i * drow.d;
+            const uint32_t wl = rem >> wshift, jj = rem & (W - 1);
             const uint64_t w = first + wl, sl = dstrips.div(w);
             const uint64_t j = umin64((w - sl * dstrips.d) * W + jj, C - 1);
             const T* base = data + sl * (uint64_t)p.R * C;
@@ -396,7 +400,8 @@ k_col_diag_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<
         const uint64_t j0 = (w - sl * dstrips.d) * W;
         const V* vbase = reinterpret_cast<const V*>(data + sl * (uint64_t)R * C) +
                          umin64(j0 / E + q, row_vecs - 1);
-        T* out = scratch + (uint64_t)wl * R * W + t;
+        T* out = scratch + (uint64_t)wl * W + t;  // row i at out[i * n * W]
+        const uint64_t pitch = (uint64_t)n * W;
         const uint32_t k0 = kblk * kb, nk = min(kb, R - k0);
         const uint32_t b0 = (uint32_t)p.dR.mod((Ix)(k0 + (uint32_t)(j0 % R)));
         const uint32_t nb = nk + W - 1;
@@ -423,7 +428,7 @@ k_col_diag_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<
 #pragma unroll
                 for (uint32_t x = 0; x < UW; x++) {
                     const uint32_t kk = min(kk0 + x * rows_per_pass, nk - 1);  // past the end: the last again
-                    o[x] = (uint64_t)r2c_prepermute(p, (Ix)(k0 + kk)) * W;
+                    o[x] = (uint64_t)r2c_prepermute(p, (Ix)(k0 + kk)) * pitch;
                     v[x] = band[(kk + t) * W + t];
                 }
 #pragma unroll
@@ -439,7 +444,7 @@ k_col_diag_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<
                     const uint32_t r = r0 + x * rows_per_pass;
                     ok[x] = r < nb && r >= t && r - t < nk;
                     const uint32_t rr = ok[x] ? r : t;  // a valid band row either way
-                    o[x] = (uint64_t)p.dR.mod((Ix)(b0 + rr)) * W;
+                    o[x] = (uint64_t)p.dR.mod((Ix)(b0 + rr)) * pitch;
                     v[x] = band[(rr - t) * W + t];
                 }
 #pragma unroll
@@ -451,10 +456,12 @@ k_col_diag_to_scratch(const T* __restrict__ data, T* __restrict__ scratch, Plan<
     }
 }
 
+// data[i][j] = scratch[i][w - first][jj] over the chunk (element e as in
+// k_col_gather_to_scratch).
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_col_scratch_back(T* __restrict__ data, const T* __restrict__ scratch, uint64_t R, uint64_t C,
-                   uint32_t wshift, Div64 dstrips, uint64_t first, uint32_t total, Div32 dper) {
+                   uint32_t wshift, Div64 dstrips, uint64_t first, uint32_t total, Div32 drow) {
     constexpr uint32_t U = 8;  // elements per thread in flight
     const uint32_t W = 1u << wshift;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -465,8 +472,8 @@ k_col_scratch_back(T* __restrict__ data, const T* __restrict__ scratch, uint64_t
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t e = min(e0 + u * stride, total - 1);
-            const uint32_t wl = dper.div(e), rem = e - wl * dper.d;
-            const uint32_t i = rem >> wshift, jj = rem & (W - 1);
+            const uint32_t i = drow.div(e), rem = e - i * drow.d;
+            const uint32_t wl = rem >> wshift, jj = rem & (W - 1);
             const uint64_t w = first + wl, sl = dstrips.div(w);
             const uint64_t j = (w - sl * dstrips.d) * W + jj;
             ok[u] = j < C;
