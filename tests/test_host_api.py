@@ -57,6 +57,19 @@ def test_validation_before_any_hip_call(ipt):
     del buf_addr
 
 
+def test_memory_kind_needs_no_device(ipt):
+    """ipt_memory_kind enqueues nothing and answers without a GPU: a NULL or
+    unknown host pointer is pageable host memory, with no owning device."""
+    L = ipt.lib()
+    dev = ctypes.c_int(7)
+    assert L.ipt_memory_kind(None, ctypes.byref(dev)) == ipt.MEM_PAGEABLE and dev.value == -1
+    buf = ctypes.create_string_buffer(64)
+    dev.value = 7
+    kind = L.ipt_memory_kind(ctypes.cast(buf, ctypes.c_void_p), ctypes.byref(dev))
+    assert kind == ipt.MEM_PAGEABLE and dev.value == -1
+    assert L.ipt_memory_kind(ctypes.cast(buf, ctypes.c_void_p), None) == ipt.MEM_PAGEABLE
+
+
 def _rand_points(rng, R, C, n):
     return rng.integers(0, R, size=n), rng.integers(0, C, size=n)
 
