@@ -291,7 +291,9 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
     // Never more workgroups than can be resident at once: every member of a
     // group must be running while the others wait for it.
     uint64_t want = std::min<uint64_t>(slots, resident);
-    if (gg.grid) want = std::min<uint64_t>(want, gg.grid);  // whole groups per round
+    // packed: whole groups per round over every resident workgroup (k * g
+    // at one workgroup per CU; twice that where the LDS allows two)
+    if (gg.grid) want = std::min<uint64_t>(want, resident / gg.g * gg.g);
     const int grid = (int)std::max<uint64_t>(1, want);
     if ((uint64_t)grid < gg.g) return IPT_ERR_UNSUPPORTED;  // a group must be co-resident
     if (gg.g == 1) {  // whole slices per workgroup: no barrier state needed

@@ -119,7 +119,7 @@ struct GcsGeom {
     uint32_t K;       // tiles per member (max)
     uint32_t KL;      // of which staged in LDS; the rest stay in registers
     size_t lds;       // bytes per workgroup
-    uint32_t grid;    // workgroups to launch; 0: as many as slots and residency allow
+    uint32_t grid;    // packed: k * g, whole groups on one workgroup per CU (0: not packed)
     bool ok;
 };
 
