@@ -1,7 +1,7 @@
 """Autotuner sweeps, selector training and evaluation.
 
     python 213transpose_amd/autotune.py sweep --count 22000 --seed 5 \
-        --start 0 --stop 5000 --out gpurun_out/tune_part0.npz
+        --start 0 --stop 5000 --out tune_part0.npz
     python 213transpose_amd/autotune.py merge --out data.npz part0.npz part1.npz ...
     python 213transpose_amd/autotune.py train --data data.npz --heldout 2000 \
         --out 213transpose_amd/selector/weights.f32
