@@ -24,7 +24,9 @@ The JSON line also carries:
                 t(per-call) / t(batch).
   cpu_baseline  the oracle's OpenMP C2R/R2C path (a port: the reference's
                 OpenMP transpose is declared but never implemented) on the same
-                two tensors, on this host's cores, bounded to ~10 s.
+                two tensors, on this host's cores, bounded to ~10 s; beside it
+                (seq_tt_serial) the oracle's out-of-place closed form on one
+                core.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --profile [--cands K2A,K2B] ...
@@ -207,10 +209,23 @@ def cpu_baseline(oracle, budget_s=10.0):
         if time.perf_counter() - t_begin > budget_s or len(times) >= 30:
             break
     med = sorted(times)[len(times) // 2]
+    # the oracle's closed form (seq_tt, out of place) on one core, as a
+    # second CPU reference point: 3 steps
+    st = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for a, (d1, d2, d3, esz) in zip(arrays, WORKLOAD):
+            oracle.seq_tt_213(a, d1, d2, d3)
+        st.append(time.perf_counter() - t0)
+    smed = sorted(st)[1]
     return {"value": round(total_bytes / med / 1e9, 3), "unit": "GB/s", "cores": threads,
             "kind": "port",
             "sample": f"{len(times)} full steps (both tensors) of the oracle's OpenMP C2R/R2C "
-                      f"path, median step {med * 1e3:.1f} ms"}
+                      f"path, median step {med * 1e3:.1f} ms",
+            "seq_tt_serial": {"value": round(total_bytes / smed / 1e9, 3), "unit": "GB/s",
+                              "cores": 1,
+                              "sample": f"3 steps of the oracle's out-of-place seq_tt, median "
+                                        f"{smed * 1e3:.1f} ms"}}
 
 
 def make_tensors(torch):
