@@ -180,19 +180,25 @@ def kernels_of(ipt, d1, d2, d3, esz, tuned=None):
     return v, out
 
 
-def time_calls(torch, fn, tensor, reps):
+def time_calls(torch, fn, tensor, reps, blocks=5):
+    """Mean time per call (ms) of `reps` back-to-back calls, timed with
+    events on the launch stream; the median over `blocks` such blocks, so a
+    single hiccup of the box does not set the figure."""
     stream = torch.cuda.current_stream()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     for _ in range(2):
         fn(tensor)
     torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(reps):
-        fn(tensor)
-    end.record(stream)
-    end.synchronize()
-    return start.elapsed_time(end) / reps
+    per_block = []
+    for _ in range(blocks):
+        start.record(stream)
+        for _ in range(reps):
+            fn(tensor)
+        end.record(stream)
+        end.synchronize()
+        per_block.append(start.elapsed_time(end) / reps)
+    return sorted(per_block)[len(per_block) // 2]
 
 
 def cpu_baseline(oracle, budget_s=10.0):
