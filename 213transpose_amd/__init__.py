@@ -7,7 +7,8 @@ ctypes and mirrors the reference's interface (include/transpose.h):
     transpose(data, d1, d2, d3)        inplace::transpose<T>      (src/transpose.cu:99-124)
     c2r.transpose(data, d1, d2, d3)    inplace::c2r::transpose<T> (src/transpose.cu:30-47)
     r2c.transpose(data, d1, d2, d3)    inplace::r2c::transpose<T> (src/transpose.cu:70-89)
-    openmp_style(row_major, data, m, n) inplace::openmp::transpose (include/openmp.h:7-8)
+    openmp_style(row_major, data, m, n) inplace::openmp::transpose (include/openmp.h:7-8), GPU
+    cpu_transpose(row_major, a, m, n)   inplace::openmp::transpose on the host CPU
 
 `data` is a contiguous CUDA (HIP) tensor of 4- or 8-byte elements holding d3
 slices of d2 rows x d1 columns; every slice is replaced by its transpose.  Work
@@ -135,6 +136,8 @@ def lib():
         getattr(L, name).argtypes = [_vp, _vp]
     L.ipt_batch_get_info.restype = _int
     L.ipt_batch_get_info.argtypes = [_vp, ctypes.POINTER(BatchInfo)]
+    L.ipt_batch_get_routes.restype = _int
+    L.ipt_batch_get_routes.argtypes = [_vp, _vp]
     L.ipt_batch_check.restype = _int
     L.ipt_batch_check.argtypes = [_vp]
     L.ipt_batch_destroy.restype = _int
@@ -165,6 +168,12 @@ def lib():
     L.ipt_selector_active.argtypes = []
     L.ipt_selector_load.restype = _int
     L.ipt_selector_load.argtypes = [_vp]
+    L.ipt_cpu_transpose.restype = _int
+    L.ipt_cpu_transpose.argtypes = [_int, _vp, _int, _i64, _i64, _vp, _int]
+    L.ipt_cpu_transpose_213.restype = _int
+    L.ipt_cpu_transpose_213.argtypes = [_vp, _int, _i64, _i64, _i64, _vp, _int]
+    L.ipt_cpu_scratch_bytes.restype = _i64
+    L.ipt_cpu_scratch_bytes.argtypes = [_int, _i64, _i64, _i64, _int]
     L.ipt_memory_kind.restype = _int
     L.ipt_memory_kind.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.ipt_version.restype = ctypes.c_char_p
@@ -268,6 +277,41 @@ def openmp_style(row_major, data, m, n, stream=None):
     return data
 
 
+def _host_args(a, n):
+    import numpy as np
+    if not isinstance(a, np.ndarray) or not a.flags["C_CONTIGUOUS"]:
+        raise TransposeError(1, "cpu path: data must be a contiguous numpy array")
+    if a.itemsize not in (4, 8):
+        raise TransposeError(1, f"element size {a.itemsize} not supported (4 or 8 bytes)")
+    if a.size != n:
+        raise TransposeError(1, f"dims give {n} elements, array has {a.size}")
+    return ctypes.c_void_p(a.ctypes.data), a.itemsize
+
+
+def cpu_transpose(row_major, a, m, n, threads=0, tmp=None):
+    """inplace::openmp::transpose(row_major, data, m, n, tmp) on the host
+    (ipt_cpu_transpose): `a` is a numpy array of m * n 4- or 8-byte
+    elements, transposed in place; `tmp` optional caller scratch (a numpy
+    array of at least cpu_scratch_bytes(...) bytes)."""
+    ptr, esz = _host_args(a, m * n)
+    t = ctypes.c_void_p(tmp.ctypes.data) if tmp is not None else None
+    _check(lib().ipt_cpu_transpose(int(bool(row_major)), ptr, esz, m, n, t, threads),
+           "ipt_cpu_transpose")
+    return a
+
+
+def cpu_transpose_213(a, d1, d2, d3=1, threads=0, tmp=None):
+    """The 213 transpose on the host (ipt_cpu_transpose_213)."""
+    ptr, esz = _host_args(a, d1 * d2 * d3)
+    t = ctypes.c_void_p(tmp.ctypes.data) if tmp is not None else None
+    _check(lib().ipt_cpu_transpose_213(ptr, esz, d1, d2, d3, t, threads), "ipt_cpu_transpose_213")
+    return a
+
+
+def cpu_scratch_bytes(esz, d1, d2, d3=1, threads=0):
+    return lib().ipt_cpu_scratch_bytes(esz, d1, d2, d3, threads)
+
+
 def transpose_variant(data, d1, d2, d3, variant, tune=None, stream=None):
     """Force one variant (and optional Tune knobs)."""
     ptr, esz = _args(data, d1, d2, d3)
@@ -317,6 +361,12 @@ def copy(dst, src, stream=None):
     return dst
 
 
+# ipt_batch_route: how a batch runs each item.
+ROUTE_SKIP, ROUTE_SLICE, ROUTE_GCS, ROUTE_SWAP, ROUTE_BLOCK_CYCLE, ROUTE_SEPARATE = range(6)
+ROUTE_NAMES = {ROUTE_SKIP: "skip", ROUTE_SLICE: "slice", ROUTE_GCS: "gcs", ROUTE_SWAP: "swap",
+               ROUTE_BLOCK_CYCLE: "block_cycle", ROUTE_SEPARATE: "separate"}
+
+
 class Batch:
     """A planned batch of uneven in-place transposes (ipt_batch_*).
 
@@ -335,6 +385,7 @@ class Batch:
             ptr, esz = _args(t, d1, d2, d3)
             descs[k] = MatrixDesc(ptr.value, esz, it[4] if len(it) > 4 else VARIANT_AUTO, d1, d2, d3)
         self._h = ctypes.c_void_p()
+        self._count = len(items)
         tptr = ctypes.byref(tune) if tune is not None else None
         _check(lib().ipt_batch_create(descs, len(items), tptr, ctypes.byref(self._h)),
                "ipt_batch_create")
@@ -347,6 +398,13 @@ class Batch:
         inf = BatchInfo()
         _check(lib().ipt_batch_get_info(self._h, ctypes.byref(inf)), "ipt_batch_get_info")
         return {f: getattr(inf, f) for f, _ in BatchInfo._fields_}
+
+    def routes(self):
+        """Route of each item (ROUTE_* values, item order)."""
+        import numpy as np
+        out = np.zeros(max(1, self._count), dtype=np.int32)
+        _check(lib().ipt_batch_get_routes(self._h, out.ctypes.data), "ipt_batch_get_routes")
+        return out[:self._count]
 
     def check(self):
         """Synchronise and raise if a group barrier of this batch gave up waiting."""

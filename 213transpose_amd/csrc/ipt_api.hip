@@ -31,6 +31,8 @@ void set_error(const char* what, hipError_t e) {
     snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
 }
 
+void set_message(const char* what) { snprintf(g_last_error, sizeof(g_last_error), "%s", what); }
+
 // ---------------------------------------------------------------- devices
 
 static std::mutex g_ctx_mu;
@@ -71,6 +73,7 @@ ipt_status get_ctx(DeviceCtx** out) {
     }
     c->pageable = pageable != 0;
     c->sync.zero_at_rest = true;
+    (void)watchdog_word(&c->wd_host, &c->sync.wd_addr);
     c->lds_max = (int64_t)prop.sharedMemPerBlock;
     snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
     *out = c.get();
@@ -78,17 +81,68 @@ ipt_status get_ctx(DeviceCtx** out) {
     return IPT_OK;
 }
 
+bool watchdog_word(uint32_t** host, uint64_t* dev_addr) {
+    *host = nullptr;
+    *dev_addr = 0;
+    void* h = nullptr;
+    void* d = nullptr;
+    if (hipHostMalloc(&h, 64, hipHostMallocMapped) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipHostFree(h);
+        return false;
+    }
+    __atomic_store_n((uint32_t*)h, 0u, __ATOMIC_RELEASE);
+    *host = (uint32_t*)h;
+    *dev_addr = (uint64_t)(uintptr_t)d;
+    return true;
+}
+
+hipError_t ctl_init(uint32_t* ctl, uint64_t wd_addr) {
+    const uint32_t w[2] = {(uint32_t)wd_addr, (uint32_t)(wd_addr >> 32)};
+    return hipMemcpy(ctl + 6, w, sizeof(w), hipMemcpyHostToDevice);
+}
+
+static bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs != hipStreamCaptureStatusNone;
+}
+
 // Grow-only arena.  Growth synchronises the device once (the old buffer may
-// still be in use by queued kernels) and is not allowed during stream capture.
+// still be in use by queued kernels) and is refused during stream capture
+// (IPT_ERR_UNSUPPORTED: run the shape once outside the capture first).  A
+// buffer a capture has used is retired rather than freed (its graph may be
+// replayed later); growth at least doubles, so retired buffers stay below
+// the live one in total.
 ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr) {
+    const bool capturing = stream_capturing(s);
     if (a.bytes < bytes) {
+        if (capturing) {
+            snprintf(g_last_error, sizeof(g_last_error),
+                     "library arena must grow to %zu bytes, not allowed during stream capture: "
+                     "run this shape once before capturing it", bytes);
+            return IPT_ERR_UNSUPPORTED;
+        }
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) { set_error("hipDeviceSynchronize", e); return IPT_ERR_HIP; }
-        if (a.ptr) (void)hipFree(a.ptr);
+        if (a.ptr) {
+            if (a.captured) a.retired.push_back(a.ptr);
+            else (void)hipFree(a.ptr);
+        }
+        const size_t old = a.bytes;
         a.ptr = nullptr;
         a.bytes = 0;
-        size_t want = a.zero_at_rest ? std::max(bytes, (size_t)1 << 20)
-                                     : std::max(bytes, (size_t)64 << 20);
+        a.captured = false;
+        a.ev_valid = false;
+        size_t want = std::max(bytes, 2 * old);
+        want = a.zero_at_rest ? std::max(want, (size_t)1 << 20) : std::max(want, (size_t)64 << 20);
         e = hipMalloc(&a.ptr, want);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -103,12 +157,14 @@ ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr) {
         }
         if (a.zero_at_rest) {
             e = hipMemset(a.ptr, 0, want);
+            if (e == hipSuccess && a.wd_addr) e = ctl_init((uint32_t*)a.ptr, a.wd_addr);
             if (e == hipSuccess) e = hipDeviceSynchronize();
             if (e != hipSuccess) { set_error("hipMemset(arena)", e); return IPT_ERR_HIP; }
         }
         a.bytes = want;
         a.last = s;
     }
+    if (capturing) a.captured = true;
     if (a.ev_valid && a.last != s) {
         hipError_t e = hipStreamWaitEvent(s, a.ev, 0);
         if (e != hipSuccess) { set_error("hipStreamWaitEvent", e); return IPT_ERR_HIP; }
@@ -117,17 +173,43 @@ ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr) {
     return IPT_OK;
 }
 
+// Zero the barrier arena (after the device went idle) and rewrite its fixed
+// control words; clears the host watchdog word.  Called with ctx->mu held.
+static hipError_t reset_sync_arena(DeviceCtx* ctx) {
+    hipError_t e = hipSuccess;
+    if (ctx->sync.ptr) {
+        e = hipMemset(ctx->sync.ptr, 0, ctx->sync.bytes);
+        if (e == hipSuccess && ctx->sync.wd_addr) e = ctl_init((uint32_t*)ctx->sync.ptr, ctx->sync.wd_addr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    if (ctx->wd_host) __atomic_store_n(ctx->wd_host, 0u, __ATOMIC_RELEASE);
+    return e;
+}
+
+ipt_status watchdog_check(DeviceCtx* ctx) {
+    if (!ctx->wd_host || __atomic_load_n(ctx->wd_host, __ATOMIC_ACQUIRE) == 0) return IPT_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (__atomic_load_n(ctx->wd_host, __ATOMIC_ACQUIRE) == 0) return IPT_OK;  // handled meanwhile
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = reset_sync_arena(ctx);
+    if (e != hipSuccess) {
+        set_error("watchdog reset", e);
+        return IPT_ERR_HIP;
+    }
+    snprintf(g_last_error, sizeof(g_last_error),
+             "a cooperative transpose enqueued earlier on device %d gave up waiting at a group "
+             "barrier (watchdog): its output is not valid; this call did nothing", ctx->dev);
+    return IPT_ERR_HIP;
+}
+
 // Event to attach to the next launch that uses the arena on stream s (the
 // launch completes it, as arena_release's record would); null while s is
 // being captured, where the caller records with arena_release instead.
 ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev) {
     *ev = nullptr;
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    hipError_t e = hipStreamIsCapturing(s, &cs);
-    if (e != hipSuccess) { set_error("hipStreamIsCapturing", e); return IPT_ERR_HIP; }
-    if (cs != hipStreamCaptureStatusNone) return IPT_OK;
+    if (stream_capturing(s)) return IPT_OK;
     if (!a.ev) {
-        e = hipEventCreateWithFlags(&a.ev, hipEventDisableTiming);
+        hipError_t e = hipEventCreateWithFlags(&a.ev, hipEventDisableTiming);
         if (e != hipSuccess) { set_error("hipEventCreate", e); return IPT_ERR_HIP; }
     }
     a.ev_valid = true;
@@ -297,6 +379,7 @@ ipt_status ipt_transpose_variant(void* data, int elem_bytes, int64_t d1, int64_t
     if (variant < 0 || variant >= IPT_VARIANT_COUNT) return IPT_ERR_INVALID_ARG;
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    if ((st = watchdog_check(ctx)) != IPT_OK) return st;
     const size_t bytes = (size_t)d1 * (size_t)d2 * (size_t)d3 * (size_t)elem_bytes;
     if ((st = prepare_memory(ctx, data, bytes, (hipStream_t)stream, true)) != IPT_OK) return st;
     Shape s = make_shape(elem_bytes, d1, d2, d3);
@@ -420,6 +503,7 @@ ipt_status ipt_permute(void* data, int elem_bytes, int rank, const int64_t* dims
     DeviceCtx* ctx;
     st = get_ctx(&ctx);
     if (st != IPT_OK) return st;
+    if ((st = watchdog_check(ctx)) != IPT_OK) return st;
     st = prepare_memory(ctx, data, (size_t)total * (size_t)elem_bytes, (hipStream_t)stream, true);
     if (st != IPT_OK) return st;
     std::vector<int> cur = start;
@@ -456,7 +540,7 @@ ipt_status ipt_plan_query(int elem_bytes, int64_t d1, int64_t d2, int64_t d3, ip
     const bool c2r = d1 >= d2;
     HostPlan h = HostPlan::make(c2r ? d2 : d1, c2r ? d1 : d2);
     out->R = h.R; out->C = h.C; out->c = h.c; out->a = h.a; out->b = h.b; out->k = h.k; out->q = h.q;
-    out->index_bits = h.fits_index32() ? 32 : 64;
+    out->index_bits = h.fits_index32() ? 32 : h.fits_index64() ? 64 : 0;
     DeviceCtx* ctx = nullptr;
     out->heuristic_variant = (get_ctx(&ctx) == IPT_OK) ? choose_variant(ctx, s)
                                                        : choose_variant(nullptr, s);
@@ -473,6 +557,7 @@ ipt_status ipt_debug_eval_map(int map_id, int64_t R, int64_t C, int index_bits, 
         return eval_map_host<uint32_t>(map_id, h, i, j, n, out);
     }
     if (index_bits != 64) return IPT_ERR_INVALID_ARG;
+    if (!h.fits_index64()) return IPT_ERR_TOO_LARGE;
     return eval_map_host<uint64_t>(map_id, h, i, j, n, out);
 }
 
@@ -483,6 +568,7 @@ ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, 
     if (map_id < 0 || map_id > MAP_R2C_SHUFFLE) return IPT_ERR_INVALID_ARG;
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    if ((st = watchdog_check(ctx)) != IPT_OK) return st;
     st = prepare_memory(ctx, data, (size_t)R * (size_t)C * (size_t)d3 * (size_t)elem_bytes,
                         (hipStream_t)stream, true);
     if (st != IPT_OK) return st;
@@ -511,10 +597,10 @@ ipt_status ipt_device_check(void) {
     uint32_t flag = 0;
     e = hipMemcpy(&flag, (uint32_t*)ctx->sync.ptr + 2, sizeof(flag), hipMemcpyDeviceToHost);
     if (e != hipSuccess) { set_error("hipMemcpy(sync flag)", e); return IPT_ERR_HIP; }
-    if (flag) {
+    const bool host_flag = ctx->wd_host && __atomic_load_n(ctx->wd_host, __ATOMIC_ACQUIRE) != 0;
+    if (flag || host_flag) {
         // A group barrier gave up waiting: restore the zero-at-rest state.
-        e = hipMemset(ctx->sync.ptr, 0, ctx->sync.bytes);
-        if (e == hipSuccess) e = hipDeviceSynchronize();
+        (void)reset_sync_arena(ctx);
         snprintf(g_last_error, sizeof(g_last_error), "group barrier watchdog tripped");
         return IPT_ERR_HIP;
     }

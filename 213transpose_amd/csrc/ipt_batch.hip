@@ -87,6 +87,8 @@ struct ipt_batch_s {
     uint32_t nq = 1;            // XCD queues
     uint32_t* d_sync = nullptr;  // 32 control words + barrier state (zero at rest)
     uint64_t sync_words = 0;
+    uint32_t* wd_host = nullptr;  // host-mapped watchdog word of this batch's barriers
+    uint64_t wd_addr = 0;         // its device address (control words 6, 7)
     uint64_t nitems = 0;
     size_t lds = 0;
     int grid = 0;
@@ -114,6 +116,7 @@ struct ipt_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ipt_batch_info info{};
+    std::vector<int32_t> routes;  // ipt_batch_route per descriptor
 };
 
 namespace {
@@ -209,7 +212,35 @@ void release(ipt_batch* b) {
     if (b->d_sync) (void)hipFree(b->d_sync);
     for (auto& m : b->moves) chunk_moves_free(&m);
     if (b->d_head) (void)hipFree(b->d_head);
+    if (b->wd_host) (void)hipHostFree(b->wd_host);
     delete b;
+}
+
+// The batch's barrier state is dirty when one of its group barriers gave up
+// (the host-mapped word says so without synchronising): wait for the
+// device, zero the state, rewrite the control block's fixed words.
+ipt_status reset_batch_sync(ipt_batch* b) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemset(b->d_sync, 0, (size_t)b->sync_words * sizeof(uint32_t));
+    if (e == hipSuccess && b->wd_addr) e = ctl_init(b->d_sync, b->wd_addr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (b->wd_host) __atomic_store_n(b->wd_host, 0u, __ATOMIC_RELEASE);
+    if (e != hipSuccess) {
+        set_error("batch barrier reset", e);
+        return IPT_ERR_HIP;
+    }
+    return IPT_OK;
+}
+
+// IPT_ERR_HIP when an earlier run of this batch tripped its watchdog (its
+// output is not valid); the state is restored, so the next run is normal.
+ipt_status batch_watchdog_check(ipt_batch* b) {
+    if (!b->wd_host || __atomic_load_n(b->wd_host, __ATOMIC_ACQUIRE) == 0) return IPT_OK;
+    const ipt_status r = reset_batch_sync(b);
+    if (r != IPT_OK) return r;
+    set_message("an earlier run of this batch gave up waiting at a group barrier (watchdog): its "
+                "output is not valid; this call did nothing");
+    return IPT_ERR_HIP;
 }
 
 }  // namespace
@@ -383,6 +414,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         const Route p = plan_instance(ctx, s, forced, lds_cap, grid, b->tune, &big_variant);
         if (p == Route::Skip) {
             b->info.skipped++;
+            b->routes.push_back(IPT_ROUTE_SKIP);
             continue;
         }
         b->info.bytes += (uint64_t)s.d1 * s.d2 * s.d3 * s.esz;
@@ -392,11 +424,16 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
                 release(b);
                 return st;
             }
+            b->routes.push_back(p == Route::Slice ? IPT_ROUTE_SLICE
+                                : p == Route::Gcs ? IPT_ROUTE_GCS : IPT_ROUTE_SWAP);
             continue;
         }
         if (big_variant == IPT_VARIANT_AUTO) big_variant = choose_variant(ctx, s);
         if (big_variant == IPT_VARIANT_BLOCK_CYCLE) {
-            if (add_block_cycle(d.data, s, &st)) continue;
+            if (add_block_cycle(d.data, s, &st)) {
+                b->routes.push_back(IPT_ROUTE_BLOCK_CYCLE);
+                continue;
+            }
             if (st != IPT_OK) {
                 release(b);
                 return st;
@@ -415,6 +452,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         }
         b->big.push_back({d.data, s, big_variant, big_tune});
         b->info.n_big++;
+        b->routes.push_back(IPT_ROUTE_SEPARATE);
     }
     // Queues (see k_batch).  Heavy work first everywhere; the members of a
     // group stay consecutive.  Groups go to the least loaded XCD queue, in
@@ -482,6 +520,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     const size_t sync_bytes = b->sync_words * sizeof(uint32_t);
     if (e == hipSuccess) e = hipMalloc(&b->d_sync, sync_bytes);
     if (e == hipSuccess) e = hipMemset(b->d_sync, 0, sync_bytes);
+    if (e == hipSuccess && watchdog_word(&b->wd_host, &b->wd_addr)) e = ctl_init(b->d_sync, b->wd_addr);
     if (e == hipSuccess && !insts.empty())
         e = hipMemcpy(b->d_insts, insts.data(), insts.size() * sizeof(BatchInst), hipMemcpyHostToDevice);
     if (e == hipSuccess && !items.empty())
@@ -512,13 +551,17 @@ static ipt_status prefetch_managed(ipt_batch* b, hipStream_t st) {
 
 ipt_status ipt_batch_run_eager(ipt_batch* b, void* stream) {
     if (!b) return IPT_ERR_INVALID_ARG;
-    const ipt_status r = prefetch_managed(b, (hipStream_t)stream);
+    ipt_status r = batch_watchdog_check(b);
+    if (r != IPT_OK) return r;
+    r = prefetch_managed(b, (hipStream_t)stream);
     if (r != IPT_OK) return r;
     return enqueue(b, (hipStream_t)stream);
 }
 
 ipt_status ipt_batch_run(ipt_batch* b, void* stream) {
     if (!b) return IPT_ERR_INVALID_ARG;
+    const ipt_status wd = batch_watchdog_check(b);
+    if (wd != IPT_OK) return wd;
     if (!b->exec) {
         hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed);
@@ -551,6 +594,12 @@ ipt_status ipt_batch_get_info(const ipt_batch* b, ipt_batch_info* info) {
     return IPT_OK;
 }
 
+ipt_status ipt_batch_get_routes(const ipt_batch* b, int32_t* routes) {
+    if (!b || (!routes && !b->routes.empty())) return IPT_ERR_INVALID_ARG;
+    std::copy(b->routes.begin(), b->routes.end(), routes);
+    return IPT_OK;
+}
+
 ipt_status ipt_batch_check(ipt_batch* b) {
     if (!b) return IPT_ERR_INVALID_ARG;
     hipError_t e = hipDeviceSynchronize();
@@ -560,11 +609,10 @@ ipt_status ipt_batch_check(ipt_batch* b) {
         set_error("ipt_batch_check", e);
         return IPT_ERR_HIP;
     }
-    if (flag) {
+    if (flag || (b->wd_host && __atomic_load_n(b->wd_host, __ATOMIC_ACQUIRE))) {
         // A group barrier gave up waiting: restore the zero-at-rest state.
-        (void)hipMemset(b->d_sync, 0, (size_t)b->sync_words * sizeof(uint32_t));
-        (void)hipDeviceSynchronize();
-        set_error("group barrier watchdog tripped in a batch", hipSuccess);
+        (void)reset_batch_sync(b);
+        set_message("group barrier watchdog tripped in a batch");
         return IPT_ERR_HIP;
     }
     return IPT_OK;

@@ -346,23 +346,35 @@ ipt_status chunk_moves_create(const BlockCyclePlan& p, ChunkMoves* out) {
 // Move lists used by the per-call path, built once per (device, rc, cc, ell)
 // and kept in device memory; past kBcPlanCacheBytes the cache is emptied
 // (after a device synchronise: queued kernels may still read the lists)
-// before the next one is added.
+// before the next one is added.  Lists that a stream capture has used are
+// not freed then but retired for the rest of the process: the captured
+// graph keeps their addresses.
+struct CachedMoves {
+    ChunkMoves m;
+    bool captured = false;
+};
 static std::mutex g_plan_mu;
-static std::map<std::tuple<int, int64_t, int64_t, int64_t>, ChunkMoves> g_plans;
+static std::map<std::tuple<int, int64_t, int64_t, int64_t>, CachedMoves> g_plans;
+static std::vector<ChunkMoves> g_retired_plans;
 static size_t g_plan_bytes = 0;  // device bytes held by g_plans
 
 static ipt_status cached_moves(const DeviceCtx* ctx, const BlockCyclePlan& p, hipStream_t st,
                                ChunkMoves* out) {
     std::lock_guard<std::mutex> g(g_plan_mu);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        cs = hipStreamCaptureStatusNone;
+    }
+    const bool capturing = cs != hipStreamCaptureStatusNone;
     const auto key = std::make_tuple(ctx->dev, p.rc, p.cc, p.ell);
     auto it = g_plans.find(key);
     if (it != g_plans.end()) {
-        *out = it->second;
+        if (capturing) it->second.captured = true;
+        *out = it->second.m;
         return IPT_OK;
     }
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
-        return IPT_ERR_UNSUPPORTED;  // building the lists allocates
+    if (capturing) return IPT_ERR_UNSUPPORTED;  // building the lists allocates
     ChunkMoves m;
     const ipt_status stt = chunk_moves_create(p, &m);
     if (stt != IPT_OK) return stt;
@@ -374,11 +386,14 @@ static ipt_status cached_moves(const DeviceCtx* ctx, const BlockCyclePlan& p, hi
             set_error("hipDeviceSynchronize", e);
             return IPT_ERR_HIP;
         }
-        for (auto& kv : g_plans) chunk_moves_free(&kv.second);
+        for (auto& kv : g_plans) {
+            if (kv.second.captured) g_retired_plans.push_back(kv.second.m);
+            else chunk_moves_free(&kv.second.m);
+        }
         g_plans.clear();
         g_plan_bytes = 0;
     }
-    g_plans[key] = m;
+    g_plans[key] = CachedMoves{m, false};
     g_plan_bytes += bytes;
     *out = m;
     return IPT_OK;

@@ -533,6 +533,7 @@ static ipt_status run_decomp(DeviceCtx* ctx, T* data, const Shape& s, bool c2r, 
     // same result with the view R = d1, C = d2 (the data is its C x R form).
     const HostPlan h = c2r ? HostPlan::make(s.d2, s.d1) : HostPlan::make(s.d1, s.d2);
     const uint64_t d3 = (uint64_t)s.d3;
+    if (!h.fits_index64()) return IPT_ERR_TOO_LARGE;  // no exact plan for this view
     if (h.fits_index32())
         return c2r ? run_c2r_ix<T, uint32_t>(ctx, data, h, d3, t, st)
                    : run_r2c_ix<T, uint32_t>(ctx, data, h, d3, t, st);
@@ -581,6 +582,7 @@ static ipt_status run_pass_ix(DeviceCtx* ctx, T* data, const HostPlan& h, uint64
 ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, int64_t d3,
                     int map_id, const Tune& t, hipStream_t stream) {
     const HostPlan h = HostPlan::make(R, C);
+    if (!h.fits_index64()) return IPT_ERR_TOO_LARGE;
     if (h.fits_index32()) {
         if (esz == 4) return run_pass_ix<uint32_t, uint32_t>(ctx, (uint32_t*)data, h, d3, map_id, t, stream);
         return run_pass_ix<uint64_t, uint32_t>(ctx, (uint64_t*)data, h, d3, map_id, t, stream);

@@ -15,7 +15,12 @@ namespace ipt {
 // A device buffer owned by the library and shared by every call on one
 // device.  Users on different streams are ordered through `ev` (the next user
 // waits for the previous user's last kernel).  `zero_at_rest` buffers are
-// cleared when allocated and every kernel that uses them restores zeros.
+// cleared when allocated and every kernel that uses them restores zeros; for
+// the barrier arena, control words 6 and 7 then hold `wd_addr` (see
+// report_watchdog).  A buffer that a stream capture has used is never freed:
+// the captured graph's nodes keep its address, so growth retires it to
+// `retired` (kept until the process ends) instead; growth itself (an
+// allocation and a device synchronise) is refused during capture.
 struct Arena {
     void* ptr = nullptr;
     size_t bytes = 0;
@@ -23,6 +28,9 @@ struct Arena {
     bool ev_valid = false;
     hipStream_t last = nullptr;
     bool zero_at_rest = false;
+    bool captured = false;        // a stream capture has recorded `ptr`
+    uint64_t wd_addr = 0;         // device address of the owner's watchdog word
+    std::vector<void*> retired;   // buffers captures may still reference
 };
 
 // Per-device state: properties read on first use of the device (not at
@@ -37,7 +45,19 @@ struct DeviceCtx {
     Arena scratch;  // fallback-tier staging
     Arena sync;     // group-barrier counters (zero at rest)
     bool pageable = false;  // kernels may touch pageable host memory (HMM)
+    // Host-mapped pinned word that a group barrier of the per-call kernels
+    // sets when it gives up waiting (report_watchdog); the next call on the
+    // device reads it with a plain host load (watchdog_check).
+    uint32_t* wd_host = nullptr;
 };
+
+// Pinned host word mapped for the device (hipHostMalloc, mapped): *host is
+// the host view, *dev_addr the address kernels store to.  False when the
+// system offers no such memory (the sticky device flag still reports).
+bool watchdog_word(uint32_t** host, uint64_t* dev_addr);
+// Writes the barrier control block's fixed words (the watchdog address) after
+// the block was zeroed.  Synchronous.
+hipError_t ctl_init(uint32_t* ctl, uint64_t wd_addr);
 
 // One 213 problem: d3 slices of d2 rows x d1 columns, esz-byte elements.
 struct Shape {
@@ -59,6 +79,7 @@ constexpr int64_t kDefaultLds = 160 * 1024;       // gfx950 LDS per workgroup
 constexpr size_t kScratchCap = (size_t)256 << 20;  // fallback-tier scratch bound
 
 void set_error(const char* what, hipError_t e);
+void set_message(const char* what);  // the detail of a non-HIP failure
 
 // Return IPT_ERR_HIP (recording the call) when a launch or HIP call failed.
 #define IPT_LAUNCH_CHECK(what)                          \
@@ -80,6 +101,12 @@ void set_error(const char* what, hipError_t e);
     } while (0)
 
 ipt_status get_ctx(DeviceCtx** out);
+// IPT_ERR_HIP (with the detail in ipt_last_error) when a cooperative kernel
+// enqueued earlier on this device gave up at a group barrier: its output is
+// not valid.  Waits for the device, restores the barrier arena and clears
+// the flag, so the call after it runs normally.  A plain host load when
+// nothing tripped.
+ipt_status watchdog_check(DeviceCtx* ctx);
 ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr);
 ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev);
 ipt_status arena_release(Arena& a, hipStream_t s);

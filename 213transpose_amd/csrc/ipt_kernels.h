@@ -528,7 +528,7 @@ namespace ipt {
 // relaxed agent-scope atomics suffice.
 //
 // Sync state (device memory, zero at rest; each launch restores it):
-//   ctl[2] watchdog flag (sticky)
+//   ctl[2] watchdog flag (sticky); ctl[6..7] the host watchdog word's address
 //   cnt[bar_words(g) * S ...] the barrier lines of slice S (see
 //   group_arrive_wait); no two slices share a line.
 // The spin is bounded: after ~2^26 polls the member records ctl[2] and
@@ -620,13 +620,25 @@ struct Watchdog {
     }
 };
 
+// A barrier gave up: set the sticky device flag ctl[2] and, when ctl[6..7]
+// (low, high half; written once when the control block is set up, never
+// while a kernel runs) hold the address of the owner's host-mapped watchdog
+// word, set that too with a system-scope store, so the host sees it on its
+// next call without synchronising.
+__device__ __forceinline__ void report_watchdog(uint32_t* ctl) {
+    atomic_store_agent(&ctl[2], 1u);
+    const uint64_t a = ((uint64_t)ctl[7] << 32) | ctl[6];
+    if (a) __hip_atomic_store(reinterpret_cast<uint32_t*>(a), 1u, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __device__ __forceinline__ void group_sync(uint32_t* bar, uint32_t* ctl, uint32_t m, uint32_t g,
                                            const Watchdog& wd) {
     wait_vmem();
     __syncthreads();
     if (g > 1) {
         if (threadIdx.x == 0 && !group_arrive_wait(bar, m, g, wd.extra, wd.limit))
-            atomic_store_agent(&ctl[2], 1u);
+            report_watchdog(ctl);
         __syncthreads();
     }
 }

@@ -112,6 +112,15 @@ struct HostPlan {
         return u(R) * u(C) + u(R) < lim && u(b) * u(b) < lim && u(c) * u(R) + u(R) < lim &&
                (u(a) + 2) * u(a) < lim;
     }
+    // The same bounds at 2^63 for the 64-bit plan (Div64).  Slices up to 2^40
+    // elements pass every bound but b^2 (k times a residue mod b in the C2R
+    // shuffle): a view with b above ~3e9 has no exact plan.
+    bool fits_index64() const {
+        const unsigned __int128 lim = (unsigned __int128)1 << 63;
+        auto u = [](int64_t v) { return (unsigned __int128)v; };
+        return u(R) * u(C) + u(R) < lim && u(b) * u(b) < lim && u(c) * u(R) + u(R) < lim &&
+               (u(a) + 2) * u(a) < lim;
+    }
 };
 
 // Device-side plan: the same numbers plus reduced divisors in index type Ix.

@@ -317,6 +317,7 @@ ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
     if (!times_us || reps < 1) return IPT_ERR_INVALID_ARG;
     DeviceCtx* ctx;
     if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    if ((st = watchdog_check(ctx)) != IPT_OK) return st;
     const size_t bytes = (size_t)d1 * (size_t)d2 * (size_t)d3 * (size_t)elem_bytes;
     if ((st = prepare_memory(ctx, data, bytes, nullptr, false)) != IPT_OK) return st;
     const Shape s = make_shape(elem_bytes, d1, d2, d3);

@@ -74,6 +74,10 @@ def parse():
                     help="build the uneven batch and run its graph a few times (for a profiler)")
     ap.add_argument("--profile", action="store_true",
                     help="only run workload steps and the calibration copies (for profilers)")
+    ap.add_argument("--profile-config", default=None, metavar="NAME",
+                    help="one reference config shape (1, 2a, ..., 3d): calibration copies of "
+                         "its bytes, then warmup + steps transposes with the default dispatch "
+                         "(for profilers)")
     ap.add_argument("--configs", action="store_true",
                     help="time every reference config shape with the default dispatch (one JSON line)")
     ap.add_argument("--only", default=None,
@@ -137,6 +141,28 @@ def configs_main(args, torch, ipt):
         torch.cuda.empty_cache()
     print(json.dumps({"configs": out, "timing": "median of 5 HIP-event-timed blocks of 10 "
                                                 "back-to-back calls, default dispatch"}))
+
+
+def profile_config_main(args, torch, ipt):
+    """Calibration copies of the config's bytes, then warmup + steps
+    transposes of it with the default dispatch, all with the same dims (the
+    work of an in-place permutation does not depend on the values, so every
+    call does the same work; runs under rocprofv3)."""
+    d1, d2, d3, esz = dict(CONFIGS)[args.profile_config]
+    dt = torch.int32 if esz == 4 else torch.int64
+    t = torch.arange(d1 * d2 * d3, dtype=dt, device="cuda")
+    raw = t.view(torch.uint8)[:t.numel() * esz // 16 * 16]  # the copy moves whole 16-byte words
+    sink = torch.empty_like(raw)
+    for _ in range(args.warmup + args.steps):
+        ipt.copy(sink, raw)
+    torch.cuda.synchronize()
+    del sink, raw
+    torch.cuda.empty_cache()
+    for _ in range(args.warmup + args.steps):
+        ipt.transpose(t, d1, d2, d3)
+    torch.cuda.synchronize()
+    ipt.device_check()
+    print(f"profile run done: config {args.profile_config} {d1}x{d2}x{d3}x{esz}", flush=True)
 
 
 def golden_sha(shape):
@@ -415,6 +441,9 @@ def main():
         return
     if args.profile_batch:
         profile_batch_main(args, torch, ipt)
+        return
+    if args.profile_config:
+        profile_config_main(args, torch, ipt)
         return
     if args.configs:
         configs_main(args, torch, ipt)

@@ -98,13 +98,42 @@ ipt_status ipt_c2r_transpose(void* data, int elem_bytes, int64_t d1, int64_t d2,
 ipt_status ipt_r2c_transpose(void* data, int elem_bytes, int64_t d1, int64_t d2,
                              int64_t d3, void* stream);
 
-/* Replaces inplace::openmp::transpose(bool row_major, T* data, int m, int n,
- * T* tmp) (include/openmp.h:7-8; declared but never implemented in the
- * reference) with a GPU implementation: row-major m x n maps to
- * (d1 = n, d2 = m, d3 = 1), column-major to (d1 = m, d2 = n, d3 = 1).  The
- * reference's `tmp` argument is dropped: scratch is library-owned. */
+/* The row-/column-major interface of inplace::openmp::transpose(bool
+ * row_major, T* data, int m, int n, T* tmp) (include/openmp.h:7-8) on the GPU:
+ * row-major m x n maps to (d1 = n, d2 = m, d3 = 1), column-major to
+ * (d1 = m, d2 = n, d3 = 1); `tmp` is not needed (scratch is library-owned).
+ * The host form, which the reference's declaration implies, is
+ * ipt_cpu_transpose below. */
 ipt_status ipt_transpose(int row_major, void* data, int elem_bytes, int64_t m,
                          int64_t n, void* stream);
+
+/* ---- host CPU path ------------------------------------------------------ */
+
+/* Replaces inplace::openmp::transpose(bool row_major, T* data, int m, int n,
+ * T* tmp) (include/openmp.h:7-8; declared but never implemented in the
+ * reference): the in-place transpose of an m x n matrix in host memory, on
+ * `threads` host threads (<= 0: one per hardware thread).  Row-major m x n
+ * maps to (d1 = n, d2 = m), column-major to (d1 = m, d2 = n), as
+ * ipt_transpose.  The same decomposition as the GPU path (C2R when the rows
+ * are at least as long as the columns, else R2C; square matrices by tile-pair
+ * swaps), driven by the library's own index maps; bit-exact.  `tmp`: NULL
+ * (the call allocates its scratch and frees it before returning) or caller
+ * scratch of at least ipt_cpu_scratch_bytes(elem_bytes, d1, d2, 1, threads)
+ * bytes.  `data` must be host-accessible (pageable, pinned or managed
+ * memory); device memory is refused with IPT_ERR_INVALID_ARG.  Blocking:
+ * returns when the transpose is done.  IPT_ERR_TOO_LARGE where the 64-bit
+ * index plan is not exact (a view with b = C / gcd(R, C) above ~3e9). */
+ipt_status ipt_cpu_transpose(int row_major, void* data, int elem_bytes, int64_t m, int64_t n,
+                             void* tmp, int threads);
+
+/* The 213 form of ipt_cpu_transpose (d3 slices of d2 rows x d1 columns, each
+ * transposed in place; inplace::transpose<T>'s semantics on the host). */
+ipt_status ipt_cpu_transpose_213(void* data, int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
+                                 void* tmp, int threads);
+
+/* Bytes of caller scratch ipt_cpu_transpose_213 needs for this problem and
+ * thread count (0 when nothing moves; -1 for invalid arguments). */
+int64_t ipt_cpu_scratch_bytes(int elem_bytes, int64_t d1, int64_t d2, int64_t d3, int threads);
 
 /* Any permutation of a rank-3 tensor in place, in the oracle's convention
  * (TensorUtil::seq_tt, src/tensor_util.cpp:80-92): dims d1, d2, d3 fastest
@@ -181,7 +210,9 @@ int ipt_variant_feasible(int variant, int elem_bytes, int64_t d1, int64_t d2, in
 typedef struct {
     int64_t R, C, c, a, b, k, q;
     int32_t heuristic_variant;
-    int32_t index_bits;   /* 32 or 64: width of the in-slice index plan */
+    int32_t index_bits;   /* 32 or 64: width of the in-slice index plan; 0 when no
+                             exact plan exists (a view with b = C / c above ~3e9:
+                             the decomposition then returns IPT_ERR_TOO_LARGE) */
 } ipt_plan_info;
 ipt_status ipt_plan_query(int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
                           ipt_plan_info* out);
@@ -191,7 +222,8 @@ ipt_status ipt_plan_query(int elem_bytes, int64_t d1, int64_t d2, int64_t d3,
  * 1 c2r post, 2 r2c pre, 3 r2c post (column gathers: source row), 4 c2r
  * shuffle, 5 r2c shuffle (row gathers: source column).  index_bits selects the
  * 32- or 64-bit plan (32 only where the plan allows it).  Writes n results
- * for the points (i[t], j[t]).  Pure host code. */
+ * for the points (i[t], j[t]).  Pure host code.  IPT_ERR_TOO_LARGE at 64 bits
+ * for a view without an exact plan (index_bits 0 in ipt_plan_query). */
 ipt_status ipt_debug_eval_map(int map_id, int64_t R, int64_t C, int index_bits,
                               const int64_t* i, const int64_t* j, int64_t n,
                               int64_t* out);
@@ -330,6 +362,19 @@ ipt_status ipt_batch_run(ipt_batch* b, void* stream);
 /* The same work launched directly, without a graph. */
 ipt_status ipt_batch_run_eager(ipt_batch* b, void* stream);
 ipt_status ipt_batch_get_info(const ipt_batch* b, ipt_batch_info* info);
+
+/* Route each descriptor of the batch took (descriptor order), for
+ * breaking a batch's time down by route: */
+typedef enum {
+    IPT_ROUTE_SKIP = 0,        /* one row or one column: nothing to move */
+    IPT_ROUTE_SLICE = 1,       /* whole slices in LDS (persistent kernel) */
+    IPT_ROUTE_GCS = 2,         /* cooperative groups (persistent kernel) */
+    IPT_ROUTE_SWAP = 3,        /* square tile pairs (persistent kernel) */
+    IPT_ROUTE_BLOCK_CYCLE = 4, /* blocks in the persistent kernel + chunk step */
+    IPT_ROUTE_SEPARATE = 5     /* per-call launchers (C2R / R2C / others) */
+} ipt_batch_route;
+/* routes: `count` int32 values (the batch's descriptor count). */
+ipt_status ipt_batch_get_routes(const ipt_batch* b, int32_t* routes);
 /* Synchronises the device and reports (IPT_ERR_HIP) whether a group barrier
  * of this batch gave up waiting; the barrier state is then reset. */
 ipt_status ipt_batch_check(ipt_batch* b);

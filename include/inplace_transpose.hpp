@@ -11,17 +11,21 @@
 //                                                               include/openmp.h:7-8
 //
 // T is any 4- or 8-byte type (the reference instantiates int, long long, float,
-// double).  `data` is device memory of the current device, managed memory
-// (prefetched to the device first, as the reference does) or pinned host
-// memory; pageable host memory is refused.  Calls are
+// double).  For the GPU forms, `data` is device memory of the current device,
+// managed memory (prefetched to the device first, as the reference does) or
+// pinned host memory; pageable host memory is refused.  Calls are
 // enqueued on the null stream, like the reference; the overloads taking a
-// hipStream_t-compatible `void* stream` enqueue there instead.
+// hipStream_t-compatible `void* stream` enqueue there instead.  The openmp
+// form runs on the host CPU (ipt_cpu_transpose) on host memory, as the
+// reference's declaration implies; `tmp` is optional caller scratch (NULL, or
+// at least ipt_cpu_scratch_bytes(sizeof(T), d1, d2, 1, threads) bytes), and
+// `threads` <= 0 uses one thread per hardware thread.
 //
 // Error behaviour: the reference prints and exits on CUDA errors
 // (src/cudacheck.cu:4-10) and throws on bad divisors / old GPUs.  Here every
 // failure throws inplace::error (a std::runtime_error carrying the ipt_status);
-// nothing exits.  The openmp form runs on the GPU: the reference declares it
-// but has no implementation, and `tmp` is ignored.
+// nothing exits.  The reference declares the openmp form but never
+// implements it; here it is the library's host path.
 #pragma once
 
 #include <cstdint>
@@ -80,8 +84,9 @@ void permute3(T* data, int d1, int d2, int d3, const int (&perm)[3], void* strea
 
 namespace openmp {
 template <typename T>
-void transpose(bool row_major, T* data, int m, int n, T* /*tmp*/, void* stream = nullptr) {
-    detail::check(ipt_transpose(row_major ? 1 : 0, data, detail::elem_bytes<T>(), m, n, stream),
+void transpose(bool row_major, T* data, int m, int n, T* tmp = nullptr, int threads = 0) {
+    detail::check(ipt_cpu_transpose(row_major ? 1 : 0, data, detail::elem_bytes<T>(), m, n, tmp,
+                                    threads),
                   "inplace::openmp::transpose");
 }
 }  // namespace openmp

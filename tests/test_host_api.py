@@ -134,3 +134,21 @@ def test_eval_map_rejects_bad_input(ipt):
         ipt.eval_map(0, 4, 4, 64, [4], [0])  # i out of range
     with pytest.raises(ipt.TransposeError):
         ipt.eval_map(4, 32, 1 << 24, 32, [0], [0])  # 32-bit plan not allowed here
+
+
+def test_views_without_exact_64bit_plan_are_refused(ipt):
+    """b = C / gcd(R, C) above ~3e9 puts k * (x mod b) of the C2R shuffle past
+    2^63: the plan query reports no index width, the host map evaluation and
+    every decomposition refuse the view (no silent wrong permutation)."""
+    import ctypes
+    big = (1 << 33) + 3  # coprime to 3: b = C
+    info = ipt.plan_query(4, big, 3)
+    assert info["b"] == big and info["index_bits"] == 0
+    i = np.zeros(1, np.int64)
+    out = np.zeros(1, np.int64)
+    L = ipt.lib()
+    rc = L.ipt_debug_eval_map(ipt.MAP_C2R_SHUFFLE, 3, big, 64, i.ctypes.data, i.ctypes.data, 1,
+                              out.ctypes.data)
+    assert rc == 3  # IPT_ERR_TOO_LARGE
+    ok = (1 << 31) + 11  # b^2 < 2^63: the 64-bit plan is exact
+    assert ipt.plan_query(4, ok, 3)["index_bits"] == 64
