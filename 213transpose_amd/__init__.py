@@ -174,6 +174,13 @@ def lib():
     L.ipt_cpu_transpose_213.argtypes = [_vp, _int, _i64, _i64, _i64, _vp, _int]
     L.ipt_cpu_scratch_bytes.restype = _i64
     L.ipt_cpu_scratch_bytes.argtypes = [_int, _i64, _i64, _i64, _int]
+    L.ipt_debug_arena_info.restype = _int
+    L.ipt_debug_arena_info.argtypes = [_int, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.POINTER(_i64)]
+    L.ipt_debug_arena_grow.restype = _int
+    L.ipt_debug_arena_grow.argtypes = [_int, ctypes.c_size_t]
+    L.ipt_transpose_oop.restype = _int
+    L.ipt_transpose_oop.argtypes = [_vp, _vp, _int, _i64, _i64, _i64, _vp]
     L.ipt_memory_kind.restype = _int
     L.ipt_memory_kind.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.ipt_version.restype = ctypes.c_char_p
@@ -365,6 +372,17 @@ def copy(dst, src, stream=None):
 ROUTE_SKIP, ROUTE_SLICE, ROUTE_GCS, ROUTE_SWAP, ROUTE_BLOCK_CYCLE, ROUTE_SEPARATE = range(6)
 ROUTE_NAMES = {ROUTE_SKIP: "skip", ROUTE_SLICE: "slice", ROUTE_GCS: "gcs", ROUTE_SWAP: "swap",
                ROUTE_BLOCK_CYCLE: "block_cycle", ROUTE_SEPARATE: "separate"}
+
+
+def transpose_oop(dst, src, d1, d2, d3=1, stream=None):
+    """Out-of-place comparator (ipt_transpose_oop): dst = 213 transpose of src."""
+    sptr, esz = _args(src, d1, d2, d3)
+    dptr, esz2 = _args(dst, d1, d2, d3)
+    if esz != esz2:
+        raise TransposeError(1, "transpose_oop: element sizes differ")
+    _check(lib().ipt_transpose_oop(dptr, sptr, esz, d1, d2, d3, _stream_ptr(stream)),
+           "ipt_transpose_oop")
+    return dst
 
 
 class Batch:
@@ -588,6 +606,22 @@ def set_watchdog(polls=0, extra=0):
     """Test hook (ipt_debug_set_watchdog): group barriers give up after
     `polls` polls and wait for `extra` arrivals beyond their group."""
     _check(lib().ipt_debug_set_watchdog(polls, extra), "ipt_debug_set_watchdog")
+
+
+ARENA_SYNC, ARENA_SCRATCH = 0, 1
+
+
+def arena_info(which):
+    """(pointer, bytes, retired buffers) of a per-device arena (test hook)."""
+    p, n, r = ctypes.c_void_p(), ctypes.c_size_t(), ctypes.c_int64()
+    _check(lib().ipt_debug_arena_info(which, ctypes.byref(p), ctypes.byref(n), ctypes.byref(r)),
+           "ipt_debug_arena_info")
+    return p.value, n.value, r.value
+
+
+def arena_grow(which, nbytes):
+    """Force a per-device arena to grow to at least nbytes (test hook)."""
+    _check(lib().ipt_debug_arena_grow(which, nbytes), "ipt_debug_arena_grow")
 
 
 def version():

@@ -586,6 +586,21 @@ ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream
     return run_copy(ctx, dst, src, bytes, (hipStream_t)stream);
 }
 
+ipt_status ipt_transpose_oop(void* dst, const void* src, int elem_bytes, int64_t d1, int64_t d2,
+                             int64_t d3, void* stream) {
+    ipt_status st = validate(src, elem_bytes, d1, d2, d3);
+    if (st != IPT_OK) return st;
+    if ((st = validate(dst, elem_bytes, d1, d2, d3)) != IPT_OK) return st;
+    const size_t bytes = (size_t)d1 * (size_t)d2 * (size_t)d3 * (size_t)elem_bytes;
+    const uintptr_t a = (uintptr_t)dst, b = (uintptr_t)src;
+    if (a < b + bytes && b < a + bytes) return IPT_ERR_INVALID_ARG;  // must not overlap
+    DeviceCtx* ctx;
+    if ((st = get_ctx(&ctx)) != IPT_OK) return st;
+    if ((st = prepare_memory(ctx, dst, bytes, (hipStream_t)stream, true)) != IPT_OK) return st;
+    if ((st = prepare_memory(ctx, src, bytes, (hipStream_t)stream, true)) != IPT_OK) return st;
+    return run_oop(ctx, dst, src, make_shape(elem_bytes, d1, d2, d3), (hipStream_t)stream);
+}
+
 ipt_status ipt_device_check(void) {
     DeviceCtx* ctx;
     ipt_status st = get_ctx(&ctx);
@@ -623,6 +638,32 @@ ipt_status ipt_debug_set_watchdog(uint32_t polls, uint32_t extra) {
         return IPT_ERR_HIP;
     }
     return IPT_OK;
+}
+
+ipt_status ipt_debug_arena_info(int which, void** ptr, size_t* bytes, int64_t* retired) {
+    if ((which != 0 && which != 1) || !ptr || !bytes || !retired) return IPT_ERR_INVALID_ARG;
+    DeviceCtx* ctx;
+    ipt_status st = get_ctx(&ctx);
+    if (st != IPT_OK) return st;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const Arena& a = which == 0 ? ctx->sync : ctx->scratch;
+    *ptr = a.ptr;
+    *bytes = a.bytes;
+    *retired = (int64_t)a.retired.size();
+    return IPT_OK;
+}
+
+ipt_status ipt_debug_arena_grow(int which, size_t bytes) {
+    if (which != 0 && which != 1) return IPT_ERR_INVALID_ARG;
+    DeviceCtx* ctx;
+    ipt_status st = get_ctx(&ctx);
+    if (st != IPT_OK) return st;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    Arena& a = which == 0 ? ctx->sync : ctx->scratch;
+    void* p;
+    st = arena_acquire(a, bytes, nullptr, &p);
+    if (st != IPT_OK) return st;
+    return arena_release(a, nullptr);
 }
 
 const char* ipt_version(void) {

@@ -32,6 +32,58 @@ k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
         dst[i] = src[i];
 }
 
+// Out-of-place transpose (the comparator; the reference times cuTT's
+// out-of-place plan for it, src/test_cutt.cu:18-41): dst slice s (C rows x R
+// columns) = transpose of src slice s (R rows x C columns).  Tiles of TS x TS
+// (TS * esz = 256 bytes) staged in LDS with an odd row pitch; VEC moves
+// 16-byte vectors (R and C multiples of 16 / esz, 16-byte aligned pointers),
+// else one element per lane.  Both sides are coalesced.
+template <typename T, bool VEC>
+__global__ void __launch_bounds__(256)
+k_oop_transpose(T* __restrict__ dst, const T* __restrict__ src, uint32_t R, uint32_t C,
+                uint64_t nslices, uint32_t nti, uint32_t ntj) {
+    constexpr uint32_t TS = 256 / sizeof(T), TP = TS + 1;
+    constexpr uint32_t PER = VEC ? 16 / sizeof(T) : 1;
+    constexpr uint32_t VPR = TS / PER, RPP = 256 / VPR;  // vectors per tile row, rows per pass
+    using V = union { uint4 v; T e[16 / sizeof(T)]; };
+    __shared__ T tile[TS * TP];
+    const uint64_t per_slice = (uint64_t)nti * ntj, total = per_slice * nslices;
+    const uint32_t x = (threadIdx.x % VPR) * PER, y0 = threadIdx.x / VPR;
+    for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const uint64_t sl = w / per_slice, t = w - sl * per_slice;
+        const uint32_t r0 = (uint32_t)(t / ntj) * TS, c0 = (uint32_t)(t % ntj) * TS;
+        const T* sp = src + sl * (uint64_t)R * C;
+        T* dp = dst + sl * (uint64_t)R * C;
+        for (uint32_t y = y0; y < TS; y += RPP) {
+            const uint32_t r = r0 + y, c = c0 + x;
+            if (r < R && c < C) {
+                if constexpr (VEC) {
+                    V u;
+                    u.v = *reinterpret_cast<const uint4*>(sp + (uint64_t)r * C + c);
+#pragma unroll
+                    for (uint32_t q = 0; q < PER; q++) tile[y * TP + x + q] = u.e[q];
+                } else {
+                    tile[y * TP + x] = sp[(uint64_t)r * C + c];
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t y = y0; y < TS; y += RPP) {  // output row c0 + y, columns r0 + x ..
+            const uint32_t orow = c0 + y, ocol = r0 + x;
+            if (orow < C && ocol < R) {
+                if constexpr (VEC) {
+                    V u;
+#pragma unroll
+                    for (uint32_t q = 0; q < PER; q++) u.e[q] = tile[(x + q) * TP + y];
+                    *reinterpret_cast<uint4*>(dp + (uint64_t)orow * R + ocol) = u.v;
+                } else {
+                    dp[(uint64_t)orow * R + ocol] = tile[x * TP + y];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
 
 int64_t lds_max_of(const DeviceCtx* ctx) { return ctx ? ctx->lds_max : kDefaultLds; }
 int ncu_of(const DeviceCtx* ctx) { return ctx ? ctx->ncu : 256; }
@@ -589,6 +641,30 @@ ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, i
     }
     if (esz == 4) return run_pass_ix<uint32_t, uint64_t>(ctx, (uint32_t*)data, h, d3, map_id, t, stream);
     return run_pass_ix<uint64_t, uint64_t>(ctx, (uint64_t*)data, h, d3, map_id, t, stream);
+}
+
+template <typename T>
+static ipt_status launch_oop(DeviceCtx* ctx, T* dst, const T* src, const Shape& s, hipStream_t st) {
+    constexpr uint32_t TS = 256 / sizeof(T), PER = 16 / sizeof(T);
+    const uint32_t R = (uint32_t)s.d2, C = (uint32_t)s.d1;
+    const uint32_t nti = (R + TS - 1) / TS, ntj = (C + TS - 1) / TS;
+    const uint64_t tiles = (uint64_t)nti * ntj * (uint64_t)s.d3;
+    const int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ncu_of(ctx) * 8);
+    const bool vec = R % PER == 0 && C % PER == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0;
+    if (vec)
+        hipLaunchKernelGGL((k_oop_transpose<T, true>), dim3(grid), dim3(256), 0, st, dst, src, R, C,
+                           (uint64_t)s.d3, nti, ntj);
+    else
+        hipLaunchKernelGGL((k_oop_transpose<T, false>), dim3(grid), dim3(256), 0, st, dst, src, R, C,
+                           (uint64_t)s.d3, nti, ntj);
+    IPT_LAUNCH_CHECK("k_oop_transpose");
+    return IPT_OK;
+}
+
+ipt_status run_oop(DeviceCtx* ctx, void* dst, const void* src, const Shape& s, hipStream_t st) {
+    if (s.d1 >= ((int64_t)1 << 32) || s.d2 >= ((int64_t)1 << 32)) return IPT_ERR_TOO_LARGE;
+    if (s.esz == 4) return launch_oop<uint32_t>(ctx, (uint32_t*)dst, (const uint32_t*)src, s, st);
+    return launch_oop<uint64_t>(ctx, (uint64_t*)dst, (const uint64_t*)src, s, st);
 }
 
 ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st) {

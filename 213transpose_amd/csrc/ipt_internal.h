@@ -134,6 +134,7 @@ ipt_status run_variant(DeviceCtx* ctx, void* data, const Shape& s, int variant, 
 ipt_status run_auto(DeviceCtx* ctx, void* data, const Shape& s, const ipt_tune* tune,
                     hipStream_t stream);
 ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st);
+ipt_status run_oop(DeviceCtx* ctx, void* dst, const void* src, const Shape& s, hipStream_t st);
 ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, int64_t d3,
                     int map_id, const Tune& t, hipStream_t stream);
 

@@ -54,20 +54,31 @@ k_mlp_sum(const float* __restrict__ v, int64_t n, float scale, float* __restrict
     if (threadIdx.x == 0) out[0] = (float)(part[0] * scale);
 }
 
-// dW[o][i] = sum_r dY[r][o] * X[r][i]; db[o] = sum_r dY[r][o].
-__global__ void __launch_bounds__(256)
+// dW[o][i] = sum_r dY[r][o] * X[r][i]; db[o] = sum_r dY[r][o].  One
+// workgroup per weight (blockIdx.x = o * (in + 1) + i, i == in for the bias):
+// each thread sums the rows r = tid, tid + 256, ... and the workgroup adds
+// the 256 partial sums as a fixed tree, so the result is the same run to run.
+constexpr int kBwdBlock = 256;
+__global__ void __launch_bounds__(kBwdBlock)
 k_mlp_linear_bwd_w(const float* __restrict__ dY, const float* __restrict__ X, float* __restrict__ dW,
                    float* __restrict__ db, int64_t n, int in, int out) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= out * (in + 1)) return;
-    const int o = idx / (in + 1), i = idx % (in + 1);
+    __shared__ float part[kBwdBlock];
+    const int o = blockIdx.x / (in + 1), i = blockIdx.x % (in + 1);
     float a = 0.0f;
     if (i == in) {
-        for (int64_t r = 0; r < n; r++) a += dY[r * out + o];
-        db[o] = a;
+        for (int64_t r = threadIdx.x; r < n; r += kBwdBlock) a += dY[r * out + o];
     } else {
-        for (int64_t r = 0; r < n; r++) a = fmaf(dY[r * out + o], X[r * in + i], a);
-        dW[o * in + i] = a;
+        for (int64_t r = threadIdx.x; r < n; r += kBwdBlock) a = fmaf(dY[r * out + o], X[r * in + i], a);
+    }
+    part[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = kBwdBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (i == in) db[o] = part[0];
+        else dW[o * in + i] = part[0];
     }
 }
 
@@ -141,15 +152,15 @@ void forward(const MlpBuffers& B, const Views& P, int64_t n, hipStream_t s) {
 void backward(const MlpBuffers& B, const Views& P, int64_t n, float scale, hipStream_t s) {
     hipLaunchKernelGGL(k_mlp_mse_grad, dim3(blocks(n * kOut)), dim3(256), 0, s, B.yh, B.y, B.mask,
                        n * kOut, scale, B.dy, B.sq);
-    hipLaunchKernelGGL(k_mlp_linear_bwd_w, dim3(blocks(kOut * (kHid + 1))), dim3(256), 0, s, B.dy,
+    hipLaunchKernelGGL(k_mlp_linear_bwd_w, dim3(kOut * (kHid + 1)), dim3(kBwdBlock), 0, s, B.dy,
                        B.h2, P.gW3, P.gb3, n, kHid, kOut);
     hipLaunchKernelGGL(k_mlp_linear_bwd_x, dim3(blocks(n * kHid)), dim3(256), 0, s, B.dy, P.W3,
                        B.h2, B.dh2, n, kHid, kOut);
-    hipLaunchKernelGGL(k_mlp_linear_bwd_w, dim3(blocks(kHid * (kHid + 1))), dim3(256), 0, s, B.dh2,
+    hipLaunchKernelGGL(k_mlp_linear_bwd_w, dim3(kHid * (kHid + 1)), dim3(kBwdBlock), 0, s, B.dh2,
                        B.h1, P.gW2, P.gb2, n, kHid, kHid);
     hipLaunchKernelGGL(k_mlp_linear_bwd_x, dim3(blocks(n * kHid)), dim3(256), 0, s, B.dh2, P.W2,
                        B.h1, B.dh1, n, kHid, kHid);
-    hipLaunchKernelGGL(k_mlp_linear_bwd_w, dim3(blocks(kHid * (kFeat + 1))), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_mlp_linear_bwd_w, dim3(kHid * (kFeat + 1)), dim3(kBwdBlock), 0, s,
                        B.dh1, B.x, P.gW1, P.gb1, n, kFeat, kHid);
 }
 

@@ -241,6 +241,15 @@ ipt_status ipt_debug_run_pass(void* data, int elem_bytes, int64_t R, int64_t C, 
  * calibration reference for bandwidth and for the HBM traffic counters. */
 ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream);
 
+/* Out-of-place transpose: dst receives the 213 transpose of src (same
+ * dims and layout conventions as ipt_transpose_213; the two buffers must not
+ * overlap).  The comparator the reference times with cuTT's out-of-place
+ * plan (src/test_cutt.cu:18-41): it moves each byte once in each direction
+ * with no in-place constraint, the practical ceiling for the in-place rates.
+ * Tiled through LDS, coalesced on both sides. */
+ipt_status ipt_transpose_oop(void* dst, const void* src, int elem_bytes, int64_t d1, int64_t d2,
+                             int64_t d3, void* stream);
+
 /* Test hook for the cooperative kernels' barrier watchdog on the current
  * device: every group barrier of later standalone launches gives up after
  * `polls` polls (0: the default, ~2^26) and waits for `extra` arrivals beyond
@@ -248,6 +257,14 @@ ipt_status ipt_debug_copy(void* dst, const void* src, size_t bytes, void* stream
  * fault; ipt_device_check then reports it and clears both settings.
  * Synchronises the device. */
 ipt_status ipt_debug_set_watchdog(uint32_t polls, uint32_t extra);
+
+/* Test hooks for the library's per-device arenas (which: 0 barrier state,
+ * 1 scratch): the current buffer, its size and how many older buffers are
+ * retired (kept allocated because a stream capture used them, so graphs
+ * captured earlier stay valid after the arena grows); and a forced growth to
+ * at least `bytes` (outside stream capture; synchronises the device). */
+ipt_status ipt_debug_arena_info(int which, void** ptr, size_t* bytes, int64_t* retired);
+ipt_status ipt_debug_arena_grow(int which, size_t bytes);
 
 /* ---- autotuner and learned selector ----------------------------------- */
 /* No counterpart in the reference: its tier thresholds are magic numbers

@@ -173,3 +173,60 @@ def test_batch_rejects_bad_descriptors(ipt):
     assert L.ipt_batch_create(descs, -1, None, ctypes.byref(h)) == 1
     assert L.ipt_batch_run(None, None) == 1
     assert L.ipt_batch_destroy(None) == 1
+
+
+@pytest.mark.gpu
+def test_uneven_workload_full_4096(ipt, oracle):
+    """The uneven-batch workload at its stated size: 4096 matrices, m and n
+    log-uniform in [2, 4096] (seed 4, as the benchmark), 4- and 8-byte
+    elements, 7.57 GB of seeded random bits.  Every instance is checked
+    against the oracle after one eager run and again after two graph
+    replays (the oracle applied three times)."""
+    shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(4096, 2, 4096, seed=4)]
+    offs, nbytes = [], 0
+    for d1, d2, d3, esz in shapes:
+        offs.append(nbytes)
+        nbytes += (d1 * d2 * d3 * esz + 255) // 256 * 256
+    print(f"[case] uneven batch 4096 instances, {nbytes / 1e9:.2f} GB", flush=True)
+    host = np.empty(nbytes, dtype=np.uint8)
+    views = []
+    for k, ((d1, d2, d3, esz), off) in enumerate(zip(shapes, offs)):
+        n = d1 * d2 * d3
+        a = oracle.random_bits(n, esz, 40000 + k)
+        host[off:off + n * esz] = a.view(np.uint8)
+        views.append((off, n, esz))
+    buf = torch.from_numpy(host).cuda()
+    items = []
+    for (d1, d2, d3, esz), (off, n, _) in zip(shapes, views):
+        items.append((buf[off:off + n * esz].view(torch.int32 if esz == 4 else torch.int64),
+                      d1, d2, d3))
+    b = ipt.Batch(items)
+    info = b.info()
+    print("[batch 4096]", info, flush=True)
+    assert info["count"] == 4096 and info["n_big"] == 0
+
+    def check(times):
+        got_all = buf.cpu().numpy()
+        bad = 0
+        for (d1, d2, d3, esz), (off, n, _) in zip(shapes, views):
+            dt = np.uint32 if esz == 4 else np.uint64
+            exp = host[off:off + n * esz].view(dt)
+            for _ in range(times):
+                exp = oracle.seq_tt_213(exp, d1, d2, d3)
+            bad += not np.array_equal(got_all[off:off + n * esz].view(dt), exp)
+        return bad
+
+    b.run(graph=False)
+    torch.cuda.synchronize()
+    b.check()
+    assert check(1) == 0
+    print("[batch 4096] eager run exact", flush=True)
+    b.run(graph=True)
+    filler = torch.zeros(1 << 20, device="cuda")
+    filler += 1  # eager work between the replays
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    assert check(3) == 0
+    print("[batch 4096] two graph replays exact", flush=True)
+    b.close()

@@ -415,3 +415,21 @@ def test_config_shapes_match_reference_gen_ans(ipt, shape):
         assert sha256_dev(t) == ref, (shape, v)
         del t
         torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("esz", [4, 8])
+def test_out_of_place_comparator_matches_oracle(ipt, oracle, esz):
+    """ipt_transpose_oop (the out-of-place comparator): dst is the oracle's
+    seq_tt of src, src untouched; 16-byte and element-wise forms."""
+    for k, (d1, d2, d3) in enumerate([(512, 384, 3), (3000, 1000, 1), (4099, 4093, 1), (7, 300, 5),
+                                      (64, 64, 2), (1, 9, 2)]):
+        print(f"[case] oop d1={d1} d2={d2} d3={d3} esz={esz}", flush=True)
+        a = oracle.random_bits(d1 * d2 * d3, esz, 7000 + k)
+        src = to_dev(a)
+        dst = torch.empty_like(src)
+        ipt.transpose_oop(dst, src, d1, d2, d3)
+        torch.cuda.synchronize()
+        assert np.array_equal(to_host(dst, esz), oracle.seq_tt_213(a, d1, d2, d3)), (d1, d2, d3)
+        assert np.array_equal(to_host(src, esz), a)
+    with pytest.raises(ipt.TransposeError):
+        ipt.transpose_oop(src, src, 1, 9, 2)  # overlapping buffers are refused
