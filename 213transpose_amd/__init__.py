@@ -136,6 +136,10 @@ def lib():
         getattr(L, name).argtypes = [_vp, _vp]
     L.ipt_batch_get_info.restype = _int
     L.ipt_batch_get_info.argtypes = [_vp, ctypes.POINTER(BatchInfo)]
+    L.ipt_batch_set_timing.restype = _int
+    L.ipt_batch_set_timing.argtypes = [_vp, _int]
+    L.ipt_batch_kernel_ms.restype = _int
+    L.ipt_batch_kernel_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_float)]
     L.ipt_batch_get_routes.restype = _int
     L.ipt_batch_get_routes.argtypes = [_vp, _vp]
     L.ipt_batch_check.restype = _int
@@ -416,6 +420,16 @@ class Batch:
         inf = BatchInfo()
         _check(lib().ipt_batch_get_info(self._h, ctypes.byref(inf)), "ipt_batch_get_info")
         return {f: getattr(inf, f) for f, _ in BatchInfo._fields_}
+
+    def set_timing(self, on=True):
+        """Eager runs record events around the persistent kernel."""
+        _check(lib().ipt_batch_set_timing(self._h, int(bool(on))), "ipt_batch_set_timing")
+
+    def kernel_ms(self):
+        """Duration of the persistent kernel in the last timed eager run (ms)."""
+        ms = ctypes.c_float()
+        _check(lib().ipt_batch_kernel_ms(self._h, ctypes.byref(ms)), "ipt_batch_kernel_ms")
+        return ms.value
 
     def routes(self):
         """Route of each item (ROUTE_* values, item order)."""

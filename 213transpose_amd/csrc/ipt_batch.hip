@@ -6,6 +6,7 @@
 // batch of instances of any shapes is planned once (items, barrier state,
 // launch geometry), and every run is one graph launch.
 #include "ipt_internal.h"
+#include <hip/hip_ext.h>
 #include "ipt_kernels.h"
 #include "ipt_select.h"
 
@@ -117,6 +118,11 @@ struct ipt_batch_s {
     hipGraphExec_t exec = nullptr;
     ipt_batch_info info{};
     std::vector<int32_t> routes;  // ipt_batch_route per descriptor
+    // Timing of the persistent kernel in eager runs (ipt_batch_set_timing):
+    // events the launch itself records around k_batch on the run's stream.
+    bool timing = false;
+    hipEvent_t k0 = nullptr, k1 = nullptr;
+    bool timed = false;  // the last eager run recorded them
 };
 
 namespace {
@@ -181,12 +187,18 @@ ipt_status run_chunks(ipt_batch* b, const std::vector<ipt_batch::Chunk>& cs, hip
     return IPT_OK;
 }
 
-ipt_status enqueue(ipt_batch* b, hipStream_t st) {
+ipt_status enqueue(ipt_batch* b, hipStream_t st, bool timed = false) {
     ipt_status r = run_chunks(b, b->pre, st);
     if (r != IPT_OK) return r;
     if (b->nitems) {
-        hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBatchBlock), b->lds, st, b->d_insts,
-                           b->d_items, (const uint32_t*)b->d_seg, b->nq, b->d_sync, b->d_sync + 32);
+        if (timed)
+            hipExtLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBatchBlock), b->lds, st, b->k0, b->k1,
+                                  0, b->d_insts, b->d_items, (const uint32_t*)b->d_seg, b->nq,
+                                  b->d_sync, b->d_sync + 32);
+        else
+            hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBatchBlock), b->lds, st, b->d_insts,
+                               b->d_items, (const uint32_t*)b->d_seg, b->nq, b->d_sync,
+                               b->d_sync + 32);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             set_error("k_batch", e);
@@ -213,6 +225,8 @@ void release(ipt_batch* b) {
     for (auto& m : b->moves) chunk_moves_free(&m);
     if (b->d_head) (void)hipFree(b->d_head);
     if (b->wd_host) (void)hipHostFree(b->wd_host);
+    if (b->k0) (void)hipEventDestroy(b->k0);
+    if (b->k1) (void)hipEventDestroy(b->k1);
     delete b;
 }
 
@@ -555,7 +569,36 @@ ipt_status ipt_batch_run_eager(ipt_batch* b, void* stream) {
     if (r != IPT_OK) return r;
     r = prefetch_managed(b, (hipStream_t)stream);
     if (r != IPT_OK) return r;
-    return enqueue(b, (hipStream_t)stream);
+    const bool timed = b->timing && b->nitems;
+    r = enqueue(b, (hipStream_t)stream, timed);
+    b->timed = timed && r == IPT_OK;
+    return r;
+}
+
+ipt_status ipt_batch_set_timing(ipt_batch* b, int on) {
+    if (!b) return IPT_ERR_INVALID_ARG;
+    if (on && !b->k0) {
+        hipError_t e = hipEventCreate(&b->k0);
+        if (e == hipSuccess) e = hipEventCreate(&b->k1);
+        if (e != hipSuccess) {
+            set_error("ipt_batch_set_timing", e);
+            return IPT_ERR_HIP;
+        }
+    }
+    b->timing = on != 0;
+    return IPT_OK;
+}
+
+ipt_status ipt_batch_kernel_ms(ipt_batch* b, float* ms) {
+    if (!b || !ms) return IPT_ERR_INVALID_ARG;
+    if (!b->timed) return IPT_ERR_NOT_READY;
+    hipError_t e = hipEventSynchronize(b->k1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, b->k0, b->k1);
+    if (e != hipSuccess) {
+        set_error("ipt_batch_kernel_ms", e);
+        return IPT_ERR_HIP;
+    }
+    return IPT_OK;
 }
 
 ipt_status ipt_batch_run(ipt_batch* b, void* stream) {

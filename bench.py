@@ -1,40 +1,51 @@
-"""Benchmark of the in-place 213 transpose on one MI355X.
+"""Benchmark of the in-place transpose engine on one MI355X.
 
-Workload (one step): the 213 in-place permutes of [A=512][B=384][C=64] float32
-and [A=256][B=1000][C=17] float64 (dims fastest-first: d1=A, d2=B, d3=C), each
-a full transpose of every slice, inputs resident in device memory before the
-timed region.  value = sum over the step of 2 * bytes (each element read once
-and written once) / step time, in GB/s.
+Headline workload (one step = one launch of the planned batch's captured
+hipGraph): the uneven batch -- 4096 row-major matrices, m and n log-uniform in
+[2, 4096] (seed 4), 4- or 8-byte elements (p = 1/2), 7.57 GB of payload,
+resident in HBM (well above the 256 MB Infinity Cache).  Every matrix is
+transposed in place; value = 2 * payload bytes (each element read once and
+written once) / step time, in GB/s.
 
 The JSON line also carries:
-  roofline      the dominant kernel of the step, timed alone with HIP events on
-                the stream it runs on: algorithmic bytes per launch (2 * tensor
-                bytes: one read and one write of every element) / average
-                launch duration, against the 8 TB/s HBM peak.  `traffic` is the
-                measured bytes per launch of that kernel from two child
-                `rocprofv3 --pmc` runs (FETCH_SIZE, then WRITE_SIZE), each scaled
-                by what the same counter reads per byte for a plain copy kernel
-                of the same size.
-  copy_GBps     a plain device copy of the same tensors (2 * bytes / time): the
-                practical ceiling for any one-sweep transpose at this size.
-  batched       the uneven-batch workload (4096 row-major matrices, m and n
-                log-uniform in [2, 4096], 4- or 8-byte elements, seed 4): one
-                graph launch of the planned batch vs the plain per-call
-                heuristic loop (one ipt_transpose_213 per matrix); speedup =
-                t(per-call) / t(batch).
+  speedup       t(plain per-call heuristic loop: one ipt_transpose_213 per
+                matrix) / t(learned selector + batched graph), the metric's
+                second part, split into the batching gain (heuristic per-call
+                loop vs heuristic batch) and the learned gain (heuristic batch
+                vs learned batch).
+  verified      every matrix checked after an untimed run: iota input, output
+                compared on the device with the closed form (the iota matrix
+                transposed by torch).
+  roofline      the batch's persistent kernel k_batch: algorithmic bytes per
+                launch (2 * payload: it reads and writes every element once)
+                / its duration, from HIP events its own launch records on the
+                run's stream (eager runs, median of 5), against the 8 TB/s
+                HBM peak.  `traffic` is FETCH_SIZE + WRITE_SIZE of k_batch per
+                launch from two child `rocprofv3 --pmc` runs of
+                `bench.py --profile-batch`, each scaled by what the counter
+                reads per byte for the plain copy kernel copying the whole
+                payload in the same run.
+  copy_GBps     a plain device copy of the whole payload (one launch).
+  rows          the reference's single-tensor configs, each transposed per
+                call with the default dispatch (median of event-timed blocks),
+                each with its residency (MALL when it fits the 256 MB Infinity
+                Cache: such rates are cache-bound and are not reported as HBM
+                results), its own roofline, and the out-of-place comparator's
+                rate on the same shape (ipt_transpose_oop).
   cpu_baseline  the oracle's OpenMP C2R/R2C path (a port: the reference's
-                OpenMP transpose is declared but never implemented) on the same
-                two tensors, on this host's cores, bounded to ~10 s; beside it
-                (seq_tt_serial) the oracle's out-of-place closed form on one
-                core.
+                OpenMP transpose is declared but never implemented) over the
+                same 4096 matrices on this host's cores; beside it (cpu_product)
+                the library's own host path (ipt_cpu_transpose), matrices
+                spread over the same number of threads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-       python bench.py --profile [--cands K2A,K2B] ...
-                                       (calibration copies, then each workload
-                                        kernel alone, for running under a
-                                        profiler)
+       python bench.py --configs [--only 1,3c]     every config shape, checked
+                                                    against its gen_ans digest
+       python bench.py --profile-batch | --profile-config NAME | --profile
+                                                    (sequences for rocprofv3)
 """
 import argparse
+import concurrent.futures
 import glob
 import hashlib
 import importlib
@@ -52,28 +63,33 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = [  # (d1, d2, d3, esz)
+MALL_BYTES = 256 << 20
+METRIC = ("in-place GB/s (2×bytes/t, % of 8TB/s HBM); learned+batched vs heuristic "
+          "speedup")
+WORKLOAD = [  # (d1, d2, d3, esz): the 213 permutes of --profile (2a and 2b)
     (512, 384, 64, 4),
     (256, 1000, 17, 8),
 ]
-WORKLOAD_NAME = "213 in-place permute [512][384][64] f32 + [256][1000][17] f64"
+BATCH_COUNT, BATCH_SEED, BATCH_HI = 4096, 4, 4096
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE child runs")
-    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-rows", action="store_true", help="skip the single-config rows")
     ap.add_argument("--cands", default=None,
                     help="--profile: selector candidate per workload tensor, comma-separated")
     ap.add_argument("--profile-batch", action="store_true",
-                    help="build the uneven batch and run its graph a few times (for a profiler)")
+                    help="calibration copy of the payload, then the uneven batch's graph "
+                         "warmup + steps times (for a profiler)")
     ap.add_argument("--profile", action="store_true",
-                    help="only run workload steps and the calibration copies (for profilers)")
+                    help="2a and 2b: calibration copies, then each tensor transposed back to "
+                         "back (for profilers)")
     ap.add_argument("--profile-config", default=None, metavar="NAME",
                     help="one reference config shape (1, 2a, ..., 3d): calibration copies of "
                          "its bytes, then warmup + steps transposes with the default dispatch "
@@ -111,6 +127,7 @@ def configs_main(args, torch, ipt):
         sha = hashlib.sha256(t.cpu().numpy().tobytes()).hexdigest()
         ok = sha == golden_sha((d1, d2, d3, esz))
         ipt.transpose(t, d2, d1, d3)
+
         def per_call(a, b):
             for _ in range(3):
                 ipt.transpose(t, a, b, d3)
@@ -173,39 +190,6 @@ def golden_sha(shape):
     return None
 
 
-MAP_LABELS = {0: ("col:c2r_pre", "MapC2RPre"), 1: ("col:c2r_post", "MapC2RPost"),
-              2: ("col:r2c_pre", "MapR2CPre"), 3: ("col:r2c_post", "MapR2CPost"),
-              4: ("row:c2r_shuffle", "MapC2RShuffle"), 5: ("row:r2c_shuffle", "MapR2CShuffle")}
-
-
-def kernels_of(ipt, d1, d2, d3, esz, tuned=None):
-    """The kernels one transpose launches: (variant, [(label, fn(tensor), name match)])."""
-    info = ipt.plan_query(esz, d1, d2, d3)
-    v = info["heuristic_variant"]
-    tune = None
-    if tuned is not None:
-        v, tune = ipt.candidate(tuned["best"])
-    tname = "unsigned int" if esz == 4 else "unsigned long"
-    single = {ipt.VARIANT_SLICE_LDS: "k_slice_lds", ipt.VARIANT_SQUARE_SWAP: "k_square_swap",
-              ipt.VARIANT_GCS: "k_gcs"}
-    if v in single:
-        fn = (lambda t: ipt.transpose_variant(t, d1, d2, d3, v, tune=tune))
-        return v, [(ipt.VARIANT_NAMES[v], fn, (single[v], f"<{tname}"))]
-    # C2R views slices as R = d2 rows x C = d1 columns, R2C as R = d1, C = d2
-    R, C = (d2, d1) if v == ipt.VARIANT_C2R else (d1, d2)
-    c = math.gcd(R, C)
-    if v == ipt.VARIANT_C2R:
-        maps = ([ipt.MAP_C2R_PRE] if c > 1 else []) + [ipt.MAP_C2R_SHUFFLE, ipt.MAP_C2R_POST]
-    else:
-        maps = [ipt.MAP_R2C_PRE, ipt.MAP_R2C_SHUFFLE] + ([ipt.MAP_R2C_POST] if c > 1 else [])
-    out = []
-    for m in maps:
-        label, mname = MAP_LABELS[m]
-        out.append((label, (lambda t, m=m: ipt.run_pass(t, R, C, d3, m, tune=tune)),
-                    (f"<{tname}, ", f"ipt::{mname}>")))
-    return v, out
-
-
 def time_calls(torch, fn, tensor, reps, blocks=5):
     """Mean time per call (ms) of `reps` back-to-back calls, timed with
     events on the launch stream; the median over `blocks` such blocks, so a
@@ -227,39 +211,6 @@ def time_calls(torch, fn, tensor, reps, blocks=5):
     return sorted(per_block)[len(per_block) // 2]
 
 
-def cpu_baseline(oracle, budget_s=10.0):
-    threads = min(16, os.cpu_count() or 1)
-    arrays = [oracle.random_bits(d1 * d2 * d3, esz, 7) for d1, d2, d3, esz in WORKLOAD]
-    total_bytes = sum(2 * a.nbytes for a in arrays)
-    times = []
-    t_begin = time.perf_counter()
-    while True:
-        t0 = time.perf_counter()
-        for a, (d1, d2, d3, esz) in zip(arrays, WORKLOAD):
-            oracle.transpose_213(a, d1, d2, d3, threads=threads)
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_begin > budget_s or len(times) >= 30:
-            break
-    med = sorted(times)[len(times) // 2]
-    # the oracle's closed form (seq_tt, out of place) on one core, as a
-    # second CPU reference point: 3 steps
-    st = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        for a, (d1, d2, d3, esz) in zip(arrays, WORKLOAD):
-            oracle.seq_tt_213(a, d1, d2, d3)
-        st.append(time.perf_counter() - t0)
-    smed = sorted(st)[1]
-    return {"value": round(total_bytes / med / 1e9, 3), "unit": "GB/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{len(times)} full steps (both tensors) of the oracle's OpenMP C2R/R2C "
-                      f"path, median step {med * 1e3:.1f} ms",
-            "seq_tt_serial": {"value": round(total_bytes / smed / 1e9, 3), "unit": "GB/s",
-                              "cores": 1,
-                              "sample": f"3 steps of the oracle's out-of-place seq_tt, median "
-                                        f"{smed * 1e3:.1f} ms"}}
-
-
 def make_tensors(torch):
     tensors = []
     for d1, d2, d3, esz in WORKLOAD:
@@ -269,12 +220,10 @@ def make_tensors(torch):
 
 
 def profile_main(args, torch, ipt):
-    """The per-kernel measurement's sequence and nothing else (runs under
-    rocprofv3): calibration copies of both tensors first, then each workload
-    tensor transposed back to back on its own, as the per-kernel timing does,
-    so every kernel sees the cache residency it had when it was timed.
-    --cands picks the selector candidate per tensor (the autotuned choice the
-    timed run used); without it the library's dispatch decides."""
+    """The 2a + 2b kernels' measurement sequence and nothing else (runs
+    under rocprofv3): calibration copies of both tensors first, then each
+    tensor transposed back to back on its own.  --cands picks the selector
+    candidate per tensor; without it the library's dispatch decides."""
     tensors = make_tensors(torch)
     sinks = [torch.empty_like(t) for t in tensors]
     for _ in range(args.warmup + args.steps):
@@ -309,22 +258,16 @@ def pmc_per_dispatch(db, counter):
     return acc
 
 
-def measure_traffic(match, shape_esz, cands):
-    """Bytes per launch of the kernel whose name contains `match`, from
-    FETCH_SIZE and WRITE_SIZE, each scaled per byte by the copy kernel that
-    copies the same tensor (same bytes read and written per dispatch).  The
-    profile run copies the workload tensors in turn (profile_main), so the
-    copies of tensor i are the copy dispatches i, i + n, i + 2n, ..."""
-    d1, d2, d3, esz = shape_esz
-    copy_bytes = d1 * d2 * d3 * esz
-    tname = "unsigned int" if esz == 4 else "unsigned long"
-    ti = WORKLOAD.index(tuple(shape_esz))
+def measure_traffic(match, copy_bytes, child_args):
+    """Bytes per launch of the kernel whose name contains every string in
+    `match`, from FETCH_SIZE and WRITE_SIZE in two child rocprofv3 runs of
+    this script with `child_args`; each counter is scaled per byte by the
+    copy kernel's dispatches of the same run, which copy `copy_bytes` each."""
     totals = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         out = tempfile.mkdtemp(prefix="ipt_pmc_")
-        cmd = ["timeout", "-s", "KILL", "120", "rocprofv3", "--pmc", counter, "-d", out, "-o",
-               "pmc", "--", sys.executable, os.path.abspath(__file__), "--profile", "--steps", "3",
-               "--warmup", "1", "--cands", cands]
+        cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", counter, "-d", out, "-o",
+               "pmc", "--", sys.executable, os.path.abspath(__file__)] + child_args
         r = subprocess.run(cmd, cwd=tempfile.gettempdir(), capture_output=True, text=True)
         dbs = glob.glob(os.path.join(out, "**", "*.db"), recursive=True)
         if r.returncode not in (0, 139) or not dbs:
@@ -332,17 +275,18 @@ def measure_traffic(match, shape_esz, cands):
         per = pmc_per_dispatch(dbs[0], counter)
         kern = [v for k, v in per.items() if all(part in k for part in match)]
         copies = [v for k, v in per.items() if "k_copy16" in k]
-        mine = copies[0][ti::len(WORKLOAD)] if copies else []
-        if not mine or not kern:
-            return None, f"{counter}: kernel not found in profile ({tname})"
+        if not copies or not kern:
+            return None, f"{counter}: kernel not found in profile"
         kern_mean = sum(kern[0]) / len(kern[0])
-        totals[counter] = kern_mean * (copy_bytes / (sum(mine) / len(mine)))
+        copy_mean = sum(copies[0]) / len(copies[0])
+        totals[counter] = kern_mean * (copy_bytes / copy_mean)
     return totals, None
 
 
-def uneven_items(torch, ipt, count=4096, seed=4):
-    """The uneven-batch workload: (tensor, d1, d2, d3) views into one buffer."""
-    shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(count, 2, 4096, seed=seed)]
+def uneven_items(torch, ipt, count=BATCH_COUNT, seed=BATCH_SEED):
+    """The uneven-batch workload: (tensor, d1, d2, d3) views into one buffer,
+    every instance 256-byte aligned."""
+    shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(count, 2, BATCH_HI, seed=seed)]
     offs, total = [], 0
     for d1, d2, d3, esz in shapes:
         offs.append(total)
@@ -359,8 +303,17 @@ def uneven_items(torch, ipt, count=4096, seed=4):
 
 
 def profile_batch_main(args, torch, ipt):
-    """The uneven batch's graph, run warmup + steps times (runs under rocprofv3)."""
-    buf, items, _ = uneven_items(torch, ipt)
+    """Calibration copies of the whole payload (one k_copy16 launch each),
+    then the uneven batch's graph, run warmup + steps times (runs under
+    rocprofv3)."""
+    buf, items, payload = uneven_items(torch, ipt)
+    raw = buf.view(torch.uint8)[:payload // 16 * 16]
+    sink = torch.empty_like(raw)
+    for _ in range(2):
+        ipt.copy(sink, raw)
+    torch.cuda.synchronize()
+    del sink
+    torch.cuda.empty_cache()
     b = ipt.Batch(items)
     for _ in range(args.warmup + args.steps):
         b.run(graph=True)
@@ -370,46 +323,195 @@ def profile_batch_main(args, torch, ipt):
     print("profile run done", flush=True)
 
 
-def batch_leg(torch, ipt, count=4096, seed=4, reps=3):
-    """Plain per-call heuristic dispatch vs the learned selector + one planned,
-    graph-captured batch, on the uneven-batch workload."""
-    buf, items, payload = uneven_items(torch, ipt, count, seed)
+def fill_iota(torch, items):
+    for t, d1, d2, d3 in items:
+        t.copy_(torch.arange(t.numel(), dtype=t.dtype, device="cuda"))
+
+
+def verify_items(torch, items, times):
+    """Every instance after `times` applications of its transpose to iota
+    input equals the closed form (the iota matrix transposed by torch, on
+    the device).  Returns the number of mismatching instances."""
+    bad = 0
+    for t, d1, d2, d3 in items:
+        e = torch.arange(t.numel(), dtype=t.dtype, device="cuda")
+        a, b = d1, d2
+        for _ in range(times):
+            e = e.view(d3, b, a).transpose(1, 2).contiguous().view(-1)
+            a, b = b, a
+        bad += not torch.equal(t, e)
+    return bad
+
+
+def batch_leg(torch, ipt, args):
+    """The headline: per-call heuristic loop, heuristic batch, learned batch
+    (graph); verification, k_batch roofline, plain copy of the payload."""
+    buf, items, payload = uneven_items(torch, ipt)
 
     def per_call():
         for t, d1, d2, d3 in items:
             ipt.transpose(t, d1, d2, d3)
 
-    def batched():
-        t0 = time.perf_counter()
-        b = ipt.Batch(items)
-        plan_ms = (time.perf_counter() - t0) * 1e3
+    def graph_ms(b, reps=3):
         b.run(graph=True)  # capture + first replay
         torch.cuda.synchronize()
-        ms = time_calls(torch, lambda _: b.run(graph=True), None, reps)
-        b.check()
-        info = b.info()
-        b.close()
-        return ms, plan_ms, info
+        return time_calls(torch, lambda _: b.run(graph=True), None, reps)
 
     ipt.selector_load(None)  # plain heuristic
-    ms_loop_h = time_calls(torch, lambda _: per_call(), None, reps)
-    ms_batch_h, _, _ = batched()
+    ms_loop_h = time_calls(torch, lambda _: per_call(), None, 2)
+    bh = ipt.Batch(items)
+    ms_batch_h = graph_ms(bh)
+    bh.check()
+    bh.close()
     learned = ipt.load_default_selector()
-    ms_loop_l = time_calls(torch, lambda _: per_call(), None, reps)
-    ms_batch_l, plan_ms, info = batched()
-    del items, buf
+    t0 = time.perf_counter()
+    b = ipt.Batch(items)
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    info = b.info()
+    routes = b.routes()
+
+    # verified: one untimed graph run on iota input, every instance checked
+    fill_iota(torch, items)
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    bad = verify_items(torch, items, 1)
+
+    # timed steps: warmup, then exactly `steps` graph launches between syncs
+    for _ in range(args.warmup):
+        b.run(graph=True)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        b.run(graph=True)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    b.check()
+
+    # the persistent kernel alone: events its launch records (eager runs)
+    b.set_timing(True)
+    kms = []
+    for _ in range(6):
+        b.run(graph=False)
+        kms.append(b.kernel_ms())
+    kms = sorted(kms[1:])
+    k_ms = kms[len(kms) // 2]
+    b.check()
+    b.close()
+
+    raw = buf.view(torch.uint8)[:payload // 16 * 16]
+    sink = torch.empty_like(raw)
+    copy_ms = time_calls(torch, lambda _: ipt.copy(sink, raw), None, 3)
+    copy_gbps = 2 * raw.numel() / (copy_ms * 1e-3) / 1e9
+    del sink, raw, items, buf
     torch.cuda.empty_cache()
-    return {"matrices": count, "payload_bytes": payload, "selector_loaded": learned,
-            "ms_per_call_heuristic": round(ms_loop_h, 3),
-            "ms_per_call_learned": round(ms_loop_l, 3),
-            "ms_batch_heuristic": round(ms_batch_h, 3),
-            "ms_batch_learned": round(ms_batch_l, 3),
-            "speedup": round(ms_loop_h / ms_batch_l, 2),
-            "speedup_note": "t(per-call heuristic) / t(learned + batched graph)",
-            "batch_GBps": round(2 * payload / (ms_batch_l * 1e-3) / 1e9, 1),
-            "plan_ms": round(plan_ms, 1),
-            "items": {k: info[k] for k in ("n_slice_items", "n_gcs_items", "n_swap_items",
-                                            "n_big", "grid")}}
+    by_route = {}
+    for r in routes.tolist():
+        name = ipt.ROUTE_NAMES[r]
+        by_route[name] = by_route.get(name, 0) + 1
+    return {
+        "ms_step": ms_step, "wall": wall, "payload": payload, "verified_bad": bad,
+        "k_batch_ms": k_ms, "copy_GBps": copy_gbps, "plan_ms": plan_ms, "learned": learned,
+        "ms_per_call_heuristic": ms_loop_h, "ms_batch_heuristic": ms_batch_h, "info": info,
+        "routes": by_route,
+    }
+
+
+def row_leg(torch, ipt, name, shape):
+    """One single-tensor config: default dispatch per call (median of 5
+    event-timed blocks), checked once on the device against the closed form,
+    plus the out-of-place comparator on the same shape."""
+    d1, d2, d3, esz = shape
+    dt = torch.int32 if esz == 4 else torch.int64
+    n = d1 * d2 * d3
+    t = torch.arange(n, dtype=dt, device="cuda")
+    ipt.transpose(t, d1, d2, d3)
+    e = torch.arange(n, dtype=dt, device="cuda").view(d3, d2, d1).transpose(1, 2).contiguous().view(-1)
+    ok = bool(torch.equal(t, e))
+    del e
+    reps = 10 if n * esz < (64 << 20) else 3
+    ms = time_calls(torch, lambda x: ipt.transpose(x, d1, d2, d3), t, reps)
+    dst = torch.empty_like(t)
+    oop_ms = time_calls(torch, lambda x: ipt.transpose_oop(dst, x, d1, d2, d3), t, reps)
+    ipt.device_check()
+    del dst, t
+    torch.cuda.empty_cache()
+    nb = 2 * n * esz
+    gbps = nb / (ms * 1e-3) / 1e9
+    v = ipt.candidate(ipt.select_candidate(esz, d1, d2, d3))[0]
+    mall = n * esz <= MALL_BYTES
+    sweeps = {ipt.VARIANT_BLOCK_CYCLE: 2, ipt.VARIANT_C2R: 3, ipt.VARIANT_R2C: 3}.get(v, 1)
+    return {"config": name, "shape": [d1, d2, d3, esz], "verified": ok,
+            "variant": ipt.VARIANT_NAMES[v], "ms": round(ms, 5), "GBps": round(gbps, 1),
+            "residency": "MALL (fits the 256 MB Infinity Cache)" if mall else "HBM",
+            "roofline": {"bound": "mall" if mall else "hbm", "achieved": round(gbps, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": None if mall else round(gbps / HBM_PEAK_GBS, 4),
+                         "frac_of_hbm_peak_for_reference": round(gbps / HBM_PEAK_GBS, 4),
+                         "sweeps": sweeps, "timing": "whole call, HIP events"},
+            "oop_GBps": round(nb / (oop_ms * 1e-3) / 1e9, 1)}
+
+
+def cpu_threads():
+    """Threads for the CPU baselines: OMP_NUM_THREADS when set (the CPU share
+    the host gives this job), else the cores this process may run on."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return (int(env) if env.isdigit() and int(env) > 0 else aff), aff
+
+
+def cpu_legs(ipt):
+    """The oracle's OpenMP path, and the library's host path, over the same
+    4096 matrices (host arrays, iota bit patterns)."""
+    import numpy as np
+    import oracle_ctypes as oracle
+    threads, aff = cpu_threads()
+    shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(BATCH_COUNT, 2, BATCH_HI,
+                                                             seed=BATCH_SEED)]
+    arrays = [np.arange(d1 * d2, dtype=np.uint32 if esz == 4 else np.uint64)
+              for d1, d2, d3, esz in shapes]
+    payload = sum(a.nbytes for a in arrays)
+
+    def oracle_pass():
+        for a, (d1, d2, d3, esz) in zip(arrays, shapes):
+            oracle.transpose_213(a, d1, d2, d3, threads=threads)
+
+    def product_pass():
+        with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda k: ipt.cpu_transpose_213(arrays[k], shapes[k][0], shapes[k][1], 1,
+                                                        threads=1), range(len(arrays))))
+
+    out = {}
+    for key, fn in (("oracle", oracle_pass), ("product", product_pass)):
+        times = []
+        t_begin = time.perf_counter()
+        while len(times) < 3 and (not times or time.perf_counter() - t_begin < 20.0):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+        out[key] = (sorted(times)[len(times) // 2], len(times))
+    nproc = os.cpu_count()
+    med, n = out["oracle"]
+    pmed, pn = out["product"]
+    return {"value": round(2 * payload / med / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": "port",
+            "sample": f"the whole uneven batch ({len(arrays)} matrices, {payload / 1e9:.2f} GB) transposed "
+                      f"in place by the oracle's OpenMP C2R/R2C path, {threads} OpenMP threads per "
+                      f"matrix, median of {n} passes ({med:.2f} s)",
+            "nproc": nproc, "affinity_cores": aff,
+            "cpu_product": {"value": round(2 * payload / pmed / 1e9, 3), "unit": "GB/s",
+                            "cores": threads,
+                            "sample": f"the same batch through the library's host path "
+                                      f"(ipt_cpu_transpose_213), matrices spread over {threads} "
+                                      f"threads, median of {pn} passes ({pmed:.2f} s)"}}
 
 
 def selector_eval(ipt):
@@ -423,6 +525,9 @@ def selector_eval(ipt):
     p = np.fromfile(ipt.SELECTOR_WEIGHTS, dtype=np.float32)
     res = at.evaluate_arrays(d["shapes"][-2000:], d["feats"][-2000:], d["times"][-2000:], p)
     return {k: round(v, 4) for k, v in res.items()}
+
+
+ROWS = ["2a+2b", "1", "3c", "3d", "3a"]
 
 
 def main():
@@ -448,123 +553,89 @@ def main():
     if args.configs:
         configs_main(args, torch, ipt)
         return
-    tensors = make_tensors(torch)
 
-    # One checked transpose (reference iota input vs gen_ans digest), untimed.
-    verified = True
-    for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
-        ipt.transpose(t, d1, d2, d3)
-        torch.cuda.synchronize()
-        sha = hashlib.sha256(t.cpu().numpy().tobytes()).hexdigest()
-        verified = verified and (sha == golden_sha((d1, d2, d3, esz)))
-        ipt.transpose(t, d2, d1, d3)  # back to the original layout
-    ipt.device_check()
-
-    def step():
-        for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
-            ipt.transpose(t, d1, d2, d3)
-
-    # Autotune the workload shapes (untimed): ipt_autotune measures every
-    # candidate and records the fastest for each exact shape, which
-    # IPT_VARIANT_AUTO then uses.
-    tuned = {}
-    for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
-        times, best = ipt.autotune(t, d1, d2, d3, reps=10)
-        tuned[f"{d1}x{d2}x{d3}x{esz}"] = {"best": best, "variant": ipt.VARIANT_NAMES[ipt.candidate(best)[0]],
-                                          "times_us": [round(float(x), 2) for x in times]}
-    # the autotune runs permuted the data: restore the iota layout
-    for t in tensors:
-        t.copy_(torch.arange(t.numel(), dtype=t.dtype, device="cuda"))
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = ev0.elapsed_time(ev1) / args.steps
-    step_bytes = sum(2 * t.numel() * t.element_size() for t in tensors)
-    value = step_bytes / (ms_per_step * 1e-3) / 1e9
-    ipt.device_check()
-
-    # Per-kernel timing (each kernel alone, events on the launch stream).
-    kernels = []
-    for t, (d1, d2, d3, esz) in zip(tensors, WORKLOAD):
-        variant, klist = kernels_of(ipt, d1, d2, d3, esz, tuned.get(f"{d1}x{d2}x{d3}x{esz}"))
-        alg = 2 * t.numel() * t.element_size()
-        for label, fn, match in klist:
-            ms = time_calls(torch, fn, t, 20)
-            kernels.append({"kernel": label, "shape": [d1, d2, d3, esz], "match": list(match),
-                            "variant": ipt.VARIANT_NAMES[variant], "ms": round(ms, 5),
-                            "alg_bytes": alg, "GBps": round(alg / (ms * 1e-3) / 1e9, 1)})
-    dom = max(kernels, key=lambda k: k["ms"])
-
-    # Practical ceiling: a plain copy of the same tensors.
-    sinks = [torch.empty_like(t) for t in tensors]
-    copy_ms = time_calls(torch, lambda _: [ipt.copy(s, t) for s, t in zip(sinks, tensors)],
-                         None, 20)
-    copy_gbps = step_bytes / (copy_ms * 1e-3) / 1e9
-    del sinks
+    bl = batch_leg(torch, ipt, args)
+    payload = bl["payload"]
+    value = 2 * payload / (bl["ms_step"] * 1e-3) / 1e9
+    k_gbps = 2 * payload / (bl["k_batch_ms"] * 1e-3) / 1e9
+    print(f"[bench] batch {bl['ms_step']:.3f} ms/step {value:.0f} GB/s, k_batch "
+          f"{bl['k_batch_ms']:.3f} ms", file=sys.stderr, flush=True)
 
     traffic, traffic_note = None, "not measured (--no-traffic)"
     if not args.no_traffic:
-        cands = ",".join(str(tuned[f"{d1}x{d2}x{d3}x{esz}"]["best"]) for d1, d2, d3, esz in WORKLOAD)
-        totals, err = measure_traffic(dom["match"], tuple(dom["shape"]), cands)
+        totals, err = measure_traffic(["k_batch"], payload // 16 * 16,
+                                      ["--profile-batch", "--steps", "2", "--warmup", "1"])
         if totals:
             traffic = int(totals["FETCH_SIZE"] + totals["WRITE_SIZE"])
-            traffic_note = (f"FETCH {totals['FETCH_SIZE'] / 1e6:.1f} MB + WRITE "
-                            f"{totals['WRITE_SIZE'] / 1e6:.1f} MB per launch, copy-calibrated")
+            traffic_note = (f"FETCH {totals['FETCH_SIZE'] / 1e9:.3f} GB + WRITE "
+                            f"{totals['WRITE_SIZE'] / 1e9:.3f} GB per launch, copy-calibrated "
+                            f"({traffic / (2 * payload):.3f}x the algorithmic bytes)")
         else:
             traffic_note = err
 
-    roofline = {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_note": traffic_note, "alg_bytes": dom["alg_bytes"],
-                "kernel": dom["kernel"], "shape": dom["shape"]}
+    rows = []
+    if not args.no_rows:
+        cfg = dict(CONFIGS)
+        for name in ROWS:
+            if name == "2a+2b":
+                a, b = row_leg(torch, ipt, "2a", cfg["2a"]), row_leg(torch, ipt, "2b", cfg["2b"])
+                nb = sum(2 * r["shape"][0] * r["shape"][1] * r["shape"][2] * r["shape"][3]
+                         for r in (a, b))
+                ms = a["ms"] + b["ms"]
+                rows.append({"config": "2a+2b", "GBps": round(nb / (ms * 1e-3) / 1e9, 1),
+                             "ms": round(ms, 5), "residency": a["residency"],
+                             "verified": a["verified"] and b["verified"], "parts": [a, b]})
+            else:
+                rows.append(row_leg(torch, ipt, name, cfg[name]))
+            print(f"[bench] row {rows[-1]['config']}: {rows[-1]['GBps']} GB/s", file=sys.stderr,
+                  flush=True)
 
-    batched = None if args.no_batch else batch_leg(torch, ipt)
-    sel_eval = selector_eval(ipt)
-
-    cpu = None
-    if not args.no_cpu:
-        import oracle_ctypes as oracle
-        cpu = cpu_baseline(oracle)
-
+    cpu = None if args.no_cpu else cpu_legs(ipt)
+    speedup = bl["ms_per_call_heuristic"] / bl["ms_step"]
     out = {
-        "metric": "in-place GB/s (2*bytes/t) of the 213 permute, % of 8 TB/s HBM",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 5),
+        "ms_per_step": round(bl["ms_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32+u64 (f32 / f64 payloads moved as raw bits)",
-        "data": "synthetic (iota bit patterns), device-resident",
-        "config": {"workload": WORKLOAD_NAME,
-                   "residency": "MALL-resident (85 MB < 256 MB Infinity Cache)",
+        "dtype": "u32+u64 (4- and 8-byte payloads moved as raw bits)",
+        "data": "synthetic (iota bit patterns per matrix), device-resident",
+        "config": {"workload": "uneven batch: 4096 row-major matrices, m and n log-uniform in "
+                               "[2, 4096] (seed 4), 4- or 8-byte elements, one captured hipGraph "
+                               "launch per step (learned selector + batched persistent kernel)",
+                   "matrices": BATCH_COUNT, "payload_bytes": payload,
+                   "residency": f"HBM ({payload / 1e9:.2f} GB > 256 MB Infinity Cache)",
                    "parallelism": "single GPU"},
         "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
-        "copy_GBps": round(copy_gbps, 1),
-        "verified_vs_gen_ans": verified,
-        "wall_s": round(wall, 3),
-        "roofline": roofline,
-        "kernels": kernels,
-        "batched": batched,
-        "selector_heldout": sel_eval,
-        "dispatch": "autotuned (measured per exact shape), learned selector for others"
-                    if ipt.selector_active() else "autotuned, heuristic for others",
-        "autotune": tuned,
+        "speedup": round(speedup, 2),
+        "speedup_note": "t(per-call heuristic loop) / t(learned + batched graph)",
+        "batching_gain": round(bl["ms_per_call_heuristic"] / bl["ms_batch_heuristic"], 3),
+        "learned_gain_in_batch": round(bl["ms_batch_heuristic"] / bl["ms_step"], 3),
+        "ms_per_call_heuristic": round(bl["ms_per_call_heuristic"], 3),
+        "ms_batch_heuristic": round(bl["ms_batch_heuristic"], 4),
+        "selector_loaded": bl["learned"],
+        "verified": bl["verified_bad"] == 0,
+        "verified_note": f"all {BATCH_COUNT} matrices vs the closed form after one untimed run "
+                         f"({bl['verified_bad']} mismatching)",
+        "copy_GBps": round(bl["copy_GBps"], 1),
+        "wall_s": round(bl["wall"], 3),
+        "plan_ms": round(bl["plan_ms"], 1),
+        "items": {k: bl["info"][k] for k in ("n_slice_items", "n_gcs_items", "n_swap_items",
+                                             "n_block_cycle", "n_big", "grid")},
+        "routes": bl["routes"],
+        "roofline": {"bound": "hbm", "achieved": round(k_gbps, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(k_gbps / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_note": traffic_note, "alg_bytes": 2 * payload,
+                     "kernel": "k_batch", "kernel_ms": round(bl["k_batch_ms"], 4),
+                     "timing": "HIP events recorded by the kernel's own launch on the run's "
+                               "stream, eager runs, median of 5"},
+        "rows": rows,
+        "selector_heldout": selector_eval(ipt),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -392,6 +392,13 @@ typedef enum {
 } ipt_batch_route;
 /* routes: `count` int32 values (the batch's descriptor count). */
 ipt_status ipt_batch_get_routes(const ipt_batch* b, int32_t* routes);
+/* Measurement: with timing on, every ipt_batch_run_eager has its persistent
+ * kernel's launch record two events around it on the run's stream;
+ * ipt_batch_kernel_ms waits for the last such run and gives that kernel's
+ * duration in milliseconds (IPT_ERR_NOT_READY before any timed run).  Graph
+ * runs are never timed this way. */
+ipt_status ipt_batch_set_timing(ipt_batch* b, int on);
+ipt_status ipt_batch_kernel_ms(ipt_batch* b, float* ms);
 /* Synchronises the device and reports (IPT_ERR_HIP) whether a group barrier
  * of this batch gave up waiting; the barrier state is then reset. */
 ipt_status ipt_batch_check(ipt_batch* b);
