@@ -101,7 +101,8 @@ def lib():
     L.ipt_status_string.argtypes = [_int]
     L.ipt_last_error.restype = ctypes.c_char_p
     L.ipt_last_error.argtypes = []
-    for name in ("ipt_transpose_213", "ipt_c2r_transpose", "ipt_r2c_transpose"):
+    for name in ("ipt_transpose_213", "ipt_c2r_transpose", "ipt_r2c_transpose",
+                 "ipt_transpose_213_noalloc"):
         f = getattr(L, name)
         f.restype = _int
         f.argtypes = [_vp, _int, _i64, _i64, _i64, _vp]
@@ -236,6 +237,15 @@ def transpose(data, d1, d2, d3=1, stream=None):
     """inplace::transpose<T>: transpose each of d3 slices (d2 rows x d1 cols) in place."""
     ptr, esz = _args(data, d1, d2, d3)
     _check(lib().ipt_transpose_213(ptr, esz, d1, d2, d3, _stream_ptr(stream)), "ipt_transpose_213")
+    return data
+
+
+def transpose_noalloc(data, d1, d2, d3=1, stream=None):
+    """ipt_transpose_213_noalloc: as transpose(), allocating nothing (scratch
+    in the library's static device array; overlapping calls on one device race)."""
+    ptr, esz = _args(data, d1, d2, d3)
+    _check(lib().ipt_transpose_213_noalloc(ptr, esz, d1, d2, d3, _stream_ptr(stream)),
+           "ipt_transpose_213_noalloc")
     return data
 
 

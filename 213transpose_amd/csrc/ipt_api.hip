@@ -73,7 +73,7 @@ ipt_status get_ctx(DeviceCtx** out) {
     }
     c->pageable = pageable != 0;
     c->sync.zero_at_rest = true;
-    (void)watchdog_word(&c->wd_host, &c->sync.wd_addr);
+    c->sync.wd_host = &c->wd_host;  // created with the arena's first buffer
     c->lds_max = (int64_t)prop.sharedMemPerBlock;
     snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
     *out = c.get();
@@ -122,6 +122,15 @@ static bool stream_capturing(hipStream_t s) {
 // replayed later); growth at least doubles, so retired buffers stay below
 // the live one in total.
 ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr) {
+    if (a.fixed) {
+        if (bytes > a.bytes) {
+            snprintf(g_last_error, sizeof(g_last_error),
+                     "needs %zu bytes of library scratch, the fixed buffer holds %zu", bytes, a.bytes);
+            return IPT_ERR_TOO_LARGE;
+        }
+        *ptr = a.ptr;
+        return IPT_OK;
+    }
     const bool capturing = stream_capturing(s);
     if (a.bytes < bytes) {
         if (capturing) {
@@ -156,6 +165,7 @@ ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr) {
             return IPT_ERR_OUT_OF_MEMORY;
         }
         if (a.zero_at_rest) {
+            if (a.wd_host && !*a.wd_host) (void)watchdog_word(a.wd_host, &a.wd_addr);
             e = hipMemset(a.ptr, 0, want);
             if (e == hipSuccess && a.wd_addr) e = ctl_init((uint32_t*)a.ptr, a.wd_addr);
             if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -207,7 +217,7 @@ ipt_status watchdog_check(DeviceCtx* ctx) {
 // being captured, where the caller records with arena_release instead.
 ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev) {
     *ev = nullptr;
-    if (stream_capturing(s)) return IPT_OK;
+    if (a.fixed || stream_capturing(s)) return IPT_OK;
     if (!a.ev) {
         hipError_t e = hipEventCreateWithFlags(&a.ev, hipEventDisableTiming);
         if (e != hipSuccess) { set_error("hipEventCreate", e); return IPT_ERR_HIP; }
@@ -219,6 +229,7 @@ ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev) {
 }
 
 ipt_status arena_release(Arena& a, hipStream_t s) {
+    if (a.fixed) return IPT_OK;
     if (!a.ev) {
         hipError_t e = hipEventCreateWithFlags(&a.ev, hipEventDisableTiming);
         if (e != hipSuccess) { set_error("hipEventCreate", e); return IPT_ERR_HIP; }
@@ -607,13 +618,18 @@ ipt_status ipt_device_check(void) {
     if (st != IPT_OK) return st;
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) { set_error("hipDeviceSynchronize", e); return IPT_ERR_HIP; }
+    bool na_tripped = false;
+    if ((st = noalloc_check(ctx, &na_tripped)) != IPT_OK) return st;
     std::lock_guard<std::mutex> g(ctx->mu);
-    if (!ctx->sync.ptr) return IPT_OK;
+    if (!ctx->sync.ptr) {
+        if (na_tripped) set_message("group barrier watchdog tripped (no-alloc path)");
+        return na_tripped ? IPT_ERR_HIP : IPT_OK;
+    }
     uint32_t flag = 0;
     e = hipMemcpy(&flag, (uint32_t*)ctx->sync.ptr + 2, sizeof(flag), hipMemcpyDeviceToHost);
     if (e != hipSuccess) { set_error("hipMemcpy(sync flag)", e); return IPT_ERR_HIP; }
     const bool host_flag = ctx->wd_host && __atomic_load_n(ctx->wd_host, __ATOMIC_ACQUIRE) != 0;
-    if (flag || host_flag) {
+    if (flag || host_flag || na_tripped) {
         // A group barrier gave up waiting: restore the zero-at-rest state.
         (void)reset_sync_arena(ctx);
         snprintf(g_last_error, sizeof(g_last_error), "group barrier watchdog tripped");

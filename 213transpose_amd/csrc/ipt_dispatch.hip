@@ -276,6 +276,34 @@ bool decomp_needs_scratch(const Shape& s, bool c2r, const Tune& t) {
     return C * s.esz + 16 > t.lds_cap || R * s.esz > t.lds_cap;
 }
 
+// Column strip of the LDS tier of a column pass (R rows x C columns): the
+// widest power of two within the LDS budget; wider than the budget only when
+// even a 1-column strip does not fit it.
+static int64_t col_strip_width(int64_t R, int64_t C, int64_t esz, const Tune& t) {
+    const int64_t wmax = std::min<int64_t>(std::max(t.strip, 1), t.block);
+    int64_t W = 1;
+    while (W * 2 <= C && W * 2 <= wmax) W *= 2;
+    if (W < C && W * 2 <= wmax) W *= 2;
+    while (W > 1 && R * W * esz > t.lds_bytes) W /= 2;
+    while (W > 1 && R * W * esz > t.lds_cap) W /= 2;
+    return W;
+}
+
+// Bytes of scratch the C2R (c2r) or R2C decomposition of s cannot do with
+// less (0: LDS tiers only): a 256-byte column strip of the view's R rows when
+// a column is taller than the LDS cap or the pass takes the scratch tier by
+// choice (every column map here moves 256-byte strips there), a whole row when
+// a row is longer than the LDS cap.
+int64_t decomp_scratch_need(const Shape& s, bool c2r, const Tune& t) {
+    const int64_t R = c2r ? s.d2 : s.d1, C = c2r ? s.d1 : s.d2;
+    int64_t need = 0;
+    const int64_t W = col_strip_width(R, C, s.esz, t);
+    const bool by_choice = t.opt_scratch && W < C && W * s.esz < 64;
+    if (R * s.esz > t.lds_cap || by_choice) need = std::max<int64_t>(need, R * 256);
+    if (C * s.esz + 16 > t.lds_cap) need = std::max<int64_t>(need, C * s.esz);
+    return need;
+}
+
 // ---------------------------------------------------------------- SLICE_LDS
 
 template <typename T>
@@ -334,7 +362,7 @@ using GcsKernel = void (*)(T*, uint32_t, uint32_t, uint64_t, uint32_t, uint32_t,
 
 template <typename T>
 static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& s,
-                             const GcsGeom& gg, hipStream_t st) {
+                             const GcsGeom& gg, Arena& sync_arena, hipStream_t st) {
     ipt_status stt = allow_lds(ctx, k, gg.lds);
     if (stt != IPT_OK) return stt;
     const int per_cu = occupancy(ctx, reinterpret_cast<const void*>(k), kGcsBlock, gg.lds);
@@ -357,7 +385,7 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
     void* sync;
-    stt = arena_acquire(ctx->sync, (size_t)(32 + bar_words(gg.g) * (uint64_t)s.d3) * sizeof(uint32_t),
+    stt = arena_acquire(sync_arena, (size_t)(32 + bar_words(gg.g) * (uint64_t)s.d3) * sizeof(uint32_t),
                         st, &sync);
     if (stt != IPT_OK) return stt;
     uint32_t* ctl = (uint32_t*)sync;
@@ -365,7 +393,7 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
     // event record between kernels costs ~3 us of queue time per call);
     // during stream capture it is recorded as its own node.
     hipEvent_t ev = nullptr;
-    stt = arena_event(ctx->sync, st, &ev);
+    stt = arena_event(sync_arena, st, &ev);
     if (stt != IPT_OK) return stt;
     if (ev) {
         hipExtLaunchKernelGGL(k, dim3(grid), dim3(kGcsBlock), gg.lds, st, nullptr, ev, 0, d,
@@ -378,17 +406,18 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
                        (uint32_t)s.d1, (uint64_t)s.d3, gg.g, gg.K, gg.KL, Div32::make(gg.ntj),
                        gg.ntiles, ctl, ctl + 32, (uint32_t)ctx->nxcd);
     IPT_LAUNCH_CHECK("k_gcs");
-    return arena_release(ctx->sync, st);
+    return arena_release(sync_arena, st);
 }
 
 template <typename T>
 static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t, hipStream_t st) {
     const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, 2, true);
     if (!gg.ok) return IPT_ERR_UNSUPPORTED;
+    Arena& a = t.sync ? *t.sync : ctx->sync;
     switch (gcs_jr_class(sizeof(T), gg.K, gg.KL, kGcsBlock / 64)) {
-    case 0: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 0)>, data, s, gg, st);
-    case 1: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 1)>, data, s, gg, st);
-    default: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 2)>, data, s, gg, st);
+    case 0: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 0)>, data, s, gg, a, st);
+    case 1: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 1)>, data, s, gg, a, st);
+    default: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 2)>, data, s, gg, a, st);
     }
 }
 
@@ -436,15 +465,7 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
     const int64_t R = h.R, C = h.C, esz = sizeof(T);
     const int64_t lds_max = t.lds_cap;
     if (R * esz <= lds_max) {
-        // Widest power-of-two strip within the budget; wider than the budget
-        // only when even a 1-column strip does not fit it.
-        int64_t wmax = std::min<int64_t>(std::max(t.strip, 1), t.block);
-        int64_t cap = 1;
-        while (cap * 2 <= C && cap * 2 <= wmax) cap *= 2;
-        if (cap < C && cap * 2 <= wmax) cap *= 2;
-        int64_t W = cap;
-        while (W > 1 && R * W * esz > t.lds_bytes) W /= 2;
-        while (W > 1 && R * W * esz > lds_max) W /= 2;
+        const int64_t W = col_strip_width(R, C, esz, t);
         bool by_choice = false;
         if (t.opt_scratch && W < C && W * esz < 64) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -472,12 +493,14 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
     const uint64_t nstrips = ((uint64_t)C + W - 1) / W;
     const uint64_t items = d3 * nstrips;
     const uint64_t per_item = (uint64_t)R * W;
-    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(kScratchCap / esz, (uint64_t)INT32_MAX) / per_item);
+    Arena& arena = t.scratch ? *t.scratch : ctx->scratch;
+    const uint64_t cap = arena.fixed ? arena.bytes : kScratchCap;
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(cap / esz, (uint64_t)INT32_MAX) / per_item);
     if (per_item > (uint64_t)INT32_MAX) return IPT_ERR_TOO_LARGE;  // one strip beyond 32-bit element math
     const Div64 dstrips = Div64::make(nstrips);
     std::lock_guard<std::mutex> g(ctx->mu);
     void* scr;
-    ipt_status stt = arena_acquire(ctx->scratch, (size_t)(std::min(chunk, items) * per_item * esz), st, &scr);
+    ipt_status stt = arena_acquire(arena, (size_t)(std::min(chunk, items) * per_item * esz), st, &scr);
     if (stt != IPT_OK) return stt;
     for (uint64_t first = 0; first < items; first += chunk) {
         const uint64_t n = std::min(chunk, items - first);
@@ -509,7 +532,7 @@ static ipt_status col_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
                            (const T*)scr, (uint64_t)R, (uint64_t)C, wshift, dstrips, first, total, drow);
         IPT_LAUNCH_CHECK("k_col_scratch_back");
     }
-    return arena_release(ctx->scratch, st);
+    return arena_release(arena, st);
 }
 
 // Row gather pass new[i][j] = old[i][s_i(j)] over every row of every slice.
@@ -534,11 +557,13 @@ static ipt_status row_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
         return IPT_OK;
     }
     // Rows longer than LDS: gather whole rows into scratch, copy back.
-    const uint64_t chunk = std::max<uint64_t>(1, kScratchCap / (uint64_t)row_bytes);
+    Arena& arena = t.scratch ? *t.scratch : ctx->scratch;
+    const uint64_t cap = arena.fixed ? arena.bytes : kScratchCap;
+    const uint64_t chunk = std::max<uint64_t>(1, cap / (uint64_t)row_bytes);
     std::lock_guard<std::mutex> g(ctx->mu);
     void* scr;
     ipt_status stt =
-        arena_acquire(ctx->scratch, (size_t)(std::min(chunk, nrows) * (uint64_t)row_bytes), st, &scr);
+        arena_acquire(arena, (size_t)(std::min(chunk, nrows) * (uint64_t)row_bytes), st, &scr);
     if (stt != IPT_OK) return stt;
     for (uint64_t g0 = 0; g0 < nrows; g0 += chunk) {
         const uint64_t n = std::min(chunk, nrows - g0);
@@ -550,7 +575,7 @@ static ipt_status row_op(DeviceCtx* ctx, T* data, const Plan<Ix>& p, const HostP
         IPT_HIP_CHECK(hipMemcpyAsync(data + g0 * (uint64_t)C, scr, (size_t)(elems * esz),
                                      hipMemcpyDeviceToDevice, st));
     }
-    return arena_release(ctx->scratch, st);
+    return arena_release(arena, st);
 }
 
 // ---------------------------------------------------------------- C2R / R2C

@@ -30,7 +30,13 @@ struct Arena {
     bool zero_at_rest = false;
     bool captured = false;        // a stream capture has recorded `ptr`
     uint64_t wd_addr = 0;         // device address of the owner's watchdog word
+    uint32_t** wd_host = nullptr; // where the owner keeps that word's host view
     std::vector<void*> retired;   // buffers captures may still reference
+    // Fixed: `ptr` / `bytes` are set once and never change (memory the
+    // library does not allocate); acquiring more than `bytes` fails with
+    // IPT_ERR_TOO_LARGE, and no ordering events are kept (users that
+    // overlap on one device race; the owner documents it).
+    bool fixed = false;
 };
 
 // Per-device state: properties read on first use of the device (not at
@@ -73,6 +79,10 @@ struct Tune {
     int strip;      // max column-strip width (elements)
     int grid_mult;  // workgroups per CU (x occupancy) for grid-stride kernels
     bool opt_scratch = true;  // column passes may take the scratch tier by choice
+    // Arenas of the run (null: the device's own): the barrier state of the
+    // cooperative kernel and the staging of the decomposition's scratch tiers.
+    Arena* sync = nullptr;
+    Arena* scratch = nullptr;
 };
 
 constexpr int64_t kDefaultLds = 160 * 1024;       // gfx950 LDS per workgroup
@@ -107,6 +117,9 @@ ipt_status get_ctx(DeviceCtx** out);
 // the flag, so the call after it runs normally.  A plain host load when
 // nothing tripped.
 ipt_status watchdog_check(DeviceCtx* ctx);
+// ipt_device_check's part for ipt_transpose_213_noalloc's barrier state on
+// this device: *tripped when one of its barriers gave up (then cleared).
+ipt_status noalloc_check(const DeviceCtx* ctx, bool* tripped);
 ipt_status arena_acquire(Arena& a, size_t bytes, hipStream_t s, void** ptr);
 ipt_status arena_event(Arena& a, hipStream_t s, hipEvent_t* ev);
 ipt_status arena_release(Arena& a, hipStream_t s);
@@ -162,6 +175,7 @@ int64_t slice_pitch(int64_t C);
 int64_t lds_max_of(const DeviceCtx* ctx);
 int ncu_of(const DeviceCtx* ctx);
 bool decomp_needs_scratch(const Shape& s, bool c2r, const Tune& t);
+int64_t decomp_scratch_need(const Shape& s, bool c2r, const Tune& t);
 
 // Block-cycle path (ipt_block_cycle.hip): chunks of T elements along one
 // side moved by cycle following, blocks of other x T transposed in one sweep.

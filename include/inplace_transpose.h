@@ -98,6 +98,25 @@ ipt_status ipt_c2r_transpose(void* data, int elem_bytes, int64_t d1, int64_t d2,
 ipt_status ipt_r2c_transpose(void* data, int elem_bytes, int64_t d1, int64_t d2,
                              int64_t d3, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation): the same result as ipt_transpose_213, calling no hipMalloc,
+ * hipFree or other allocating routine.  The cooperative kernel's barrier
+ * state and the decomposition's scratch staging live in one static 32 MiB
+ * device array (1 MiB barrier state, 31 MiB staging), which every device the
+ * library runs on holds until the process ends.  Routes that need
+ * allocations run differently: the block-cycle path (move lists) and
+ * cooperative groups whose barrier state exceeds 1 MiB take the
+ * decomposition, whose scratch tiers stage through the array in pieces;
+ * IPT_ERR_TOO_LARGE, before anything is enqueued, when one of those pieces
+ * (a row of the R x C view, or a 256-byte strip of all R rows) exceeds
+ * 31 MiB.  Every call on a device shares that device's array and leaves its
+ * barrier state at zero: calls that overlap on one device (several streams
+ * or host threads at once) race.  A barrier that gives up is reported by
+ * ipt_device_check. */
+ipt_status ipt_transpose_213_noalloc(void* data, int elem_bytes, int64_t d1, int64_t d2,
+                                     int64_t d3, void* stream);
+
 /* The row-/column-major interface of inplace::openmp::transpose(bool
  * row_major, T* data, int m, int n, T* tmp) (include/openmp.h:7-8) on the GPU:
  * row-major m x n maps to (d1 = n, d2 = m, d3 = 1), column-major to
