@@ -879,6 +879,40 @@ inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw) {
     return 2;
 }
 
+// Which tiles a member of a cooperative group owns.  Strided (bh == 0): member
+// m takes tiles m, m + g, m + 2g, ... (k < nk).  Banded (bh > 0): tiles are
+// walked band by band, a band being bh tile rows (the last band may be
+// shorter) walked column by column: position p = b * bh * ntj + tj * h + r is
+// tile (b * bh + r, tj), h the band's height; member m owns positions
+// m * K .. m * K + nk - 1, about a bh x (K / bh) block of adjacent tiles.
+// Adjacent tiles share the 128-byte lines that a row not aligned to 128
+// bytes splits, on the input side (along a tile row) and on the output side
+// (along a tile column); in a block those stay with one member, instead of
+// being fetched and partly written by several (on several XCDs when the
+// group spans them).  Both orders are bijections onto the tiles.
+struct TileOrder {
+    uint32_t bh = 0;    // band height in tiles; 0: strided
+    uint32_t full = 0;  // bands of height bh
+    Div32 dband{};      // bh * ntj: positions per full band
+    Div32 dbh{};        // bh
+    Div32 dlast{};      // the last, shorter band's height (unused when none)
+};
+
+// Banded order for a group whose members hold up to K tiles of a slice with
+// nti x ntj tiles: bands about sqrt(K) tiles tall, so a member's block is
+// roughly square (host side).
+inline TileOrder banded_order(uint32_t K, uint32_t nti, uint32_t ntj) {
+    TileOrder o;
+    uint32_t bh = 1;
+    while ((bh + 1) * (bh + 1) <= K) bh++;
+    o.bh = bh < nti ? bh : nti;
+    o.full = nti / o.bh;
+    o.dband = Div32::make(o.bh * ntj);
+    o.dbh = Div32::make(o.bh);
+    o.dlast = Div32::make(nti - o.full * o.bh);
+    return o;
+}
+
 // One GCS slot: member m of the group of g that transposes the
 // slice at `base`.  Member m owns tiles t = m + k*g, k < nk; wave w of the
 // workgroup takes k = w, w + NW, ...  Tiles k < kl are staged in LDS
@@ -888,21 +922,33 @@ inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw) {
 template <typename T, int BLOCK, int JR, int SP>
 __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
                                           uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
-                                          uint32_t m, uint32_t* bar, uint32_t* ctl, u32x4* lds) {
+                                          uint32_t m, uint32_t* bar, uint32_t* ctl, u32x4* lds,
+                                          const TileOrder& ord = TileOrder{}) {
     using W = WaveTile<T>;
     constexpr uint32_t NW = BLOCK / 64;
     constexpr int JL = gcs_wave_lds_tiles(sizeof(T));
     constexpr uint32_t TV = 64 * W::PER;  // 16-byte vectors per staged tile
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t nk = m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0;
+    const uint32_t nk = ord.bh == 0 ? (m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0)
+                                    : (m * K < ntiles ? min(K, ntiles - m * K) : 0);
     const uint32_t kl = min(nk, KL);
     const Rsrc rs = make_rsrc(base, R * C * (uint32_t)sizeof(T));
     Watchdog wd;
     wd.read(ctl, g);
     auto origin = [&](uint32_t k, uint32_t* r0, uint32_t* c0) {
-        const uint32_t t = m + k * g;
-        const uint32_t ti = dntj.div(t), tj = t - ti * dntj.d;
+        uint32_t ti, tj;
+        if (ord.bh == 0) {
+            const uint32_t t = m + k * g;
+            ti = dntj.div(t);
+            tj = t - ti * dntj.d;
+        } else {
+            const uint32_t p = m * K + k;
+            const uint32_t b = ord.dband.div(p), rem = p - b * ord.dband.d;
+            const Div32& dh = b < ord.full ? ord.dbh : ord.dlast;  // the band's height
+            tj = dh.div(rem);
+            ti = b * ord.dbh.d + (rem - tj * dh.d);
+        }
         *r0 = ti * W::TS;
         *c0 = tj * W::TS;
     };
@@ -1002,6 +1048,7 @@ struct BatchInst {
     uint32_t g, K, KL, ntiles;   // BK_GCS
     Div32 dntj;                  // BK_GCS
     uint32_t jr;                 // BK_GCS: register-tile class (gcs_jr_class)
+    TileOrder ord;               // BK_GCS: which tiles each member owns
     uint32_t P;                  // BK_SLICE: LDS row pitch
     Div32 dS, dC, dR;            // BK_SLICE
 };
@@ -1034,10 +1081,10 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
         static_assert(kBatchJrClass == 1, "one body per register-tile class");
         if (I.jr == 0)
             gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
-                                                              I.ntiles, it.sub, bar, ctl, l);
+                                                              I.ntiles, it.sub, bar, ctl, l, I.ord);
         else
             gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
-                                                              I.ntiles, it.sub, bar, ctl, l);
+                                                              I.ntiles, it.sub, bar, ctl, l, I.ord);
         break;
     }
     case BK_SWAP_RS:  // buffer-resource form: slices below 2^31 bytes, 16-byte rows
