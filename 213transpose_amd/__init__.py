@@ -55,7 +55,9 @@ class TransposeError(RuntimeError):
 class Tune(ctypes.Structure):
     """ipt_tune: zero fields mean library default."""
     _fields_ = [("lds_bytes", ctypes.c_int32), ("block", ctypes.c_int32),
-                ("strip", ctypes.c_int32), ("grid_mult", ctypes.c_int32)]
+                ("strip", ctypes.c_int32), ("grid_mult", ctypes.c_int32),
+                ("chunk_min", ctypes.c_int32), ("chunk_max", ctypes.c_int32),
+                ("tile", ctypes.c_int32)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -478,7 +480,7 @@ def uneven_shapes(count, lo=2, hi=4096, seed=0):
 
 # ---------------------------------------------------------------- selector
 
-N_CANDIDATES = 9
+N_CANDIDATES = 13
 MLP_IN, MLP_HIDDEN = 8, 64
 MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + \
     N_CANDIDATES * MLP_HIDDEN + N_CANDIDATES

@@ -163,7 +163,7 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
 // inside a batch than their standalone timings show).  -1 = let
 // plan_instance's heuristic decide.
 int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t lds_cap, int grid) {
-    const int tuned = tuned_candidate(s);
+    const int tuned = tuned_candidate(ctx, s);
     if (tuned >= 0) return tuned;
     if (!y || !selector_domain(s.d1, s.d2, s.d3, s.esz) || s.d1 == s.d2) return -1;
     const int64_t slice_bytes = s.d2 * slice_pitch(s.d1) * s.esz;
@@ -384,9 +384,10 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     // one chunk-matrix shape).  false = the instance cannot go this way.
     std::map<std::tuple<int64_t, int64_t, int64_t>, size_t> moves_of;
     size_t head_bytes = 0;
-    auto add_block_cycle = [&](void* data, const Shape& s, ipt_status* err) -> bool {
+    auto add_block_cycle = [&](void* data, const Shape& s, const Tune& t, ipt_status* err) -> bool {
         *err = IPT_OK;
-        const BlockCyclePlan bp = block_cycle_for(block_cycle_plan(ctx, s), data);
+        const BlockCyclePlan bp =
+            block_cycle_for(block_cycle_plan(ctx, s, t.chunk_min, t.chunk_max), data);
         if (!bp.ok) return false;
         int bv = IPT_VARIANT_AUTO;
         const Route rb = plan_instance(ctx, bp.blk, bp.block_variant, lds_cap, grid, b->tune, &bv);
@@ -425,7 +426,8 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             const int k = batch_choice(ctx, s, y, lds_cap, grid);
             if (k >= 0) {
                 forced = kCandidates[k].variant;
-                if (forced == IPT_VARIANT_C2R || forced == IPT_VARIANT_R2C)
+                if (forced == IPT_VARIANT_C2R || forced == IPT_VARIANT_R2C ||
+                    forced == IPT_VARIANT_BLOCK_CYCLE)
                     big_tune = resolve_tune(ctx, &kCandidates[k].tune);
             }
         }
@@ -449,7 +451,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         }
         if (big_variant == IPT_VARIANT_AUTO) big_variant = choose_variant(ctx, s);
         if (big_variant == IPT_VARIANT_BLOCK_CYCLE) {
-            if (add_block_cycle(d.data, s, &st)) {
+            if (add_block_cycle(d.data, s, big_tune, &st)) {
                 b->routes.push_back(IPT_ROUTE_BLOCK_CYCLE);
                 continue;
             }

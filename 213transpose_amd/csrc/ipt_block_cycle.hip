@@ -184,12 +184,12 @@ static int block_kernel(const DeviceCtx* ctx, const Shape& blk, double* eff) {
     return IPT_VARIANT_AUTO;
 }
 
-static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, BlockCyclePlan* best,
-                             double* best_cost) {
+static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, int64_t cmin,
+                             int64_t cmax, BlockCyclePlan* best, double* best_cost) {
     const int64_t esz = s.esz;
     const int64_t L = cols ? s.d1 : s.d2, other = cols ? s.d2 : s.d1;
-    const int64_t tmax = std::min<int64_t>(L / 2, kBcMaxChunk / esz);
-    const int64_t tmin = (kBcMinChunk + esz - 1) / esz;
+    const int64_t tmax = std::min<int64_t>(L / 2, cmax / esz);
+    const int64_t tmin = (cmin + esz - 1) / esz;
     for (int64_t T = tmax; T >= tmin; T--) {
         if (L % T) continue;
         const int64_t nb = L / T;
@@ -247,12 +247,16 @@ BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data) {
     return q;
 }
 
-BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s) {
+BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s, int64_t chunk_min,
+                                int64_t chunk_max) {
     BlockCyclePlan p{};
     if (s.d1 == s.d2 || s.d1 < 2 || s.d2 < 2) return p;
+    const int64_t cmin = std::max<int64_t>(chunk_min > 0 ? chunk_min : kBcMinChunk, kBcMinChunk);
+    const int64_t cmax = std::min<int64_t>(chunk_max > 0 ? chunk_max : kBcMaxChunk, kBcMaxChunk);
+    if (cmin > cmax) return p;
     double cost = 0;
-    plan_orientation(ctx, s, true, &p, &cost);
-    plan_orientation(ctx, s, false, &p, &cost);
+    plan_orientation(ctx, s, true, cmin, cmax, &p, &cost);
+    plan_orientation(ctx, s, false, cmin, cmax, &p, &cost);
     return p;
 }
 
@@ -513,7 +517,7 @@ ipt_status run_chunk_transpose(DeviceCtx* ctx, void* data, int esz, int64_t inne
 
 ipt_status run_block_cycle(DeviceCtx* ctx, void* data, const Shape& s, const Tune& t,
                            hipStream_t st) {
-    const BlockCyclePlan p = block_cycle_for(block_cycle_plan(ctx, s), data);
+    const BlockCyclePlan p = block_cycle_for(block_cycle_plan(ctx, s, t.chunk_min, t.chunk_max), data);
     if (!p.ok) return IPT_ERR_UNSUPPORTED;
     if (p.chunk_first) {
         const ipt_status stt = chunk_step(ctx, data, s, p, st);

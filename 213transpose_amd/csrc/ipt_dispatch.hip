@@ -103,6 +103,9 @@ Tune resolve_tune(const DeviceCtx* ctx, const ipt_tune* t) {
         if (t->block >= 64 && t->block <= 1024 && (t->block & (t->block - 1)) == 0) r.block = t->block;
         if (t->strip > 0) r.strip = t->strip;
         if (t->grid_mult > 0) r.grid_mult = t->grid_mult;
+        if (t->chunk_min > 0) r.chunk_min = t->chunk_min;
+        if (t->chunk_max > 0) r.chunk_max = t->chunk_max;
+        if (t->tile > 0) r.tile = t->tile;
     }
     return r;
 }
@@ -334,10 +337,11 @@ static ipt_status run_slice_lds(DeviceCtx* ctx, T* data, const Shape& s, const T
 
 // ---------------------------------------------------------------- SQUARE_SWAP
 
-template <typename T>
-static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t,
-                             hipStream_t st) {
-    constexpr int TS = sizeof(T) == 4 ? 64 : 32;
+// Square slices with tiles of TS x TS (the default 64 / 32 for 4- / 8-byte
+// elements, or half of that: the `tile` knob).
+template <typename T, int TS>
+static ipt_status run_square_ts(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t,
+                                hipStream_t st) {
     const uint32_t N = (uint32_t)s.d1;
     const uint32_t nt = (N + TS - 1) / TS;
     const uint64_t pairs = (uint64_t)nt * (nt + 1) / 2;
@@ -351,6 +355,14 @@ static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune
     hipLaunchKernelGGL(k, dim3(grid), dim3(kSwapBlock), 0, st, data, N, pairs, (uint64_t)s.d3);
     IPT_LAUNCH_CHECK("k_square_swap");
     return IPT_OK;
+}
+
+template <typename T>
+static ipt_status run_square(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t,
+                             hipStream_t st) {
+    constexpr int TS = sizeof(T) == 4 ? 64 : 32;
+    if (t.tile > 0 && t.tile <= TS / 2) return run_square_ts<T, TS / 2>(ctx, data, s, t, st);
+    return run_square_ts<T, TS>(ctx, data, s, t, st);
 }
 
 // ---------------------------------------------------------------- GCS

@@ -78,6 +78,8 @@ struct Tune {
     int block;      // threads per workgroup
     int strip;      // max column-strip width (elements)
     int grid_mult;  // workgroups per CU (x occupancy) for grid-stride kernels
+    int64_t chunk_min = 0, chunk_max = 0;  // block-cycle chunk bytes (0: the defaults)
+    int tile = 0;   // square-swap tile side (0: default)
     bool opt_scratch = true;  // column passes may take the scratch tier by choice
     // Arenas of the run (null: the device's own): the barrier state of the
     // cooperative kernel and the staging of the decomposition's scratch tiers.
@@ -140,7 +142,7 @@ int choose_variant(const DeviceCtx* ctx, const Shape& s);
 int select_candidate(const DeviceCtx* ctx, const Shape& s);
 bool selector_loaded();
 bool predict_gpu(const std::vector<Shape>& shapes, std::vector<float>& y);
-int tuned_candidate(const Shape& s);  // -1 when the shape has no measured choice
+int tuned_candidate(const DeviceCtx* ctx, const Shape& s);  // -1: no measured choice
 bool candidate_ok(const DeviceCtx* ctx, const Shape& s, int k);
 ipt_status run_variant(DeviceCtx* ctx, void* data, const Shape& s, int variant, const Tune& t,
                        hipStream_t stream);
@@ -203,7 +205,10 @@ struct BlockCyclePlan {
     Shape blk;                 // the block step as a 213 problem
     int block_variant = 0;     // its one-sweep kernel
 };
-BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s);
+// The cheapest plan with chunks of chunk_min .. chunk_max bytes (0: the
+// defaults kBcMinChunk .. kBcMaxChunk).
+BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s, int64_t chunk_min = 0,
+                                int64_t chunk_max = 0);
 // The plan for data at `data`: vectors narrowed to the data's alignment.
 BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data);
 // Parts and segment length of a plan whose T, blk.esz and vb are set.

@@ -23,17 +23,24 @@
 
 namespace ipt {
 
-// Knob order: lds_bytes (cap), block, strip, grid_mult.
+// Knob order: lds_bytes (cap), block, strip, grid_mult, chunk_min, chunk_max,
+// tile.  Candidates 9-11 split the block-cycle plan's chunk range (512 B ..
+// 32 KB, where candidate 8 lets the price model choose) into three classes;
+// candidate 12 is the square swap with half-size tiles.
 const Candidate kCandidates[kNumCandidates] = {
-    {IPT_VARIANT_SLICE_LDS, {0, 0, 0, 0}},
-    {IPT_VARIANT_SQUARE_SWAP, {0, 0, 0, 0}},
-    {IPT_VARIANT_GCS, {0, 0, 0, 0}},
-    {IPT_VARIANT_GCS, {81920, 0, 0, 0}},
-    {IPT_VARIANT_C2R, {0, 0, 0, 0}},
-    {IPT_VARIANT_C2R, {0, 512, 16, 0}},
-    {IPT_VARIANT_R2C, {0, 0, 0, 0}},
-    {IPT_VARIANT_R2C, {0, 512, 16, 0}},
-    {IPT_VARIANT_BLOCK_CYCLE, {0, 0, 0, 0}},
+    {IPT_VARIANT_SLICE_LDS, {0, 0, 0, 0, 0, 0, 0}},
+    {IPT_VARIANT_SQUARE_SWAP, {0, 0, 0, 0, 0, 0, 0}},
+    {IPT_VARIANT_GCS, {0, 0, 0, 0, 0, 0, 0}},
+    {IPT_VARIANT_GCS, {81920, 0, 0, 0, 0, 0, 0}},
+    {IPT_VARIANT_C2R, {0, 0, 0, 0, 0, 0, 0}},
+    {IPT_VARIANT_C2R, {0, 512, 16, 0, 0, 0, 0}},
+    {IPT_VARIANT_R2C, {0, 0, 0, 0, 0, 0, 0}},
+    {IPT_VARIANT_R2C, {0, 512, 16, 0, 0, 0, 0}},
+    {IPT_VARIANT_BLOCK_CYCLE, {0, 0, 0, 0, 0, 0, 0}},
+    {IPT_VARIANT_BLOCK_CYCLE, {0, 0, 0, 0, 512, 2048, 0}},
+    {IPT_VARIANT_BLOCK_CYCLE, {0, 0, 0, 0, 2049, 8192, 0}},
+    {IPT_VARIANT_BLOCK_CYCLE, {0, 0, 0, 0, 8193, 32768, 0}},
+    {IPT_VARIANT_SQUARE_SWAP, {0, 0, 0, 0, 0, 0, 1}},
 };
 
 int heuristic_candidate(int variant) {
@@ -95,7 +102,23 @@ static bool candidate_feasible(const DeviceCtx* ctx, const Shape& s, int k) {
         const int64_t cap = c.tune.lds_bytes > 0 ? c.tune.lds_bytes : lds_max_of(ctx);
         return gcs_geom(ctx, s, cap).ok;
     }
+    if (c.variant == IPT_VARIANT_BLOCK_CYCLE)
+        return block_cycle_plan(ctx, s, c.tune.chunk_min, c.tune.chunk_max).ok;
     return variant_feasible(ctx, s, c.variant);
+}
+
+// Shape classes of the tuning table: half-octave buckets of R and C, an
+// octave bucket of d3, the element size and whether the slice is square.  A
+// choice the autotuner measured on one shape is used for every shape of its
+// class (where that candidate is feasible), after exact shapes.
+using ShapeClass = std::tuple<int, int, int, int, int>;
+static int half_octave(int64_t x) {
+    const double l = std::log2((double)std::max<int64_t>(x, 1));
+    return (int)std::floor(2.0 * l + 1e-9);
+}
+static ShapeClass shape_class(const Shape& s) {
+    return std::make_tuple(s.esz, half_octave(s.d2), half_octave(s.d1),
+                           (int)std::floor(std::log2((double)s.d3) + 1e-9), s.d1 == s.d2 ? 1 : 0);
 }
 
 // ---------------------------------------------------------------- selector
@@ -105,9 +128,21 @@ static std::vector<float> g_sel_params;  // empty = no selector
 // Selector choices per (device, shape): feasibility depends on the device.
 static std::map<std::tuple<int, int, int64_t, int64_t, int64_t>, int> g_sel_cache;
 static bool g_sel_default_tried = false;
-// Measured choices: exact shape -> fastest candidate, filled by ipt_autotune
-// (and ipt_tuning_load).  Consulted before the selector.
+// Measured choices, filled by ipt_autotune (and ipt_tuning_load): exact shape
+// -> fastest candidate, and shape class -> the fastest candidate of the last
+// shape of the class tuned.  Consulted before the selector, exact first.
 static std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> g_tuned;
+static std::map<ShapeClass, int> g_tuned_class;
+
+// The measured choice for a shape (exact, else its class when feasible), or
+// -1.  Called with g_sel_mu held.
+static int tuned_locked(const DeviceCtx* ctx, const Shape& s) {
+    auto it = g_tuned.find(std::make_tuple(s.esz, s.d1, s.d2, s.d3));
+    if (it != g_tuned.end() && candidate_feasible(ctx, s, it->second)) return it->second;
+    auto ic = g_tuned_class.find(shape_class(s));
+    if (ic != g_tuned_class.end() && candidate_feasible(ctx, s, ic->second)) return ic->second;
+    return -1;
+}
 
 // The trained weights shipped with the library: $IPT_SELECTOR_WEIGHTS, else
 // selector/weights.f32 next to this shared object.  IPT_SELECTOR=heuristic
@@ -151,8 +186,8 @@ int select_candidate(const DeviceCtx* ctx, const Shape& s) {
     const bool in_domain = !square && selector_domain(s.d1, s.d2, s.d3, s.esz);
     {
         std::lock_guard<std::mutex> g(g_sel_mu);
-        auto tuned = g_tuned.find(std::make_tuple(s.esz, s.d1, s.d2, s.d3));
-        if (tuned != g_tuned.end() && candidate_feasible(ctx, s, tuned->second)) return tuned->second;
+        const int tuned = tuned_locked(ctx, s);
+        if (tuned >= 0) return tuned;
         load_default_weights_locked();
         if (!g_sel_params.empty() && in_domain) {
             auto key = std::make_tuple(ctx ? ctx->dev : -1, s.esz, s.d1, s.d2, s.d3);
@@ -193,10 +228,9 @@ bool predict_gpu(const std::vector<Shape>& shapes, std::vector<float>& y) {
     return ipt_selector_infer(params.data(), x.data(), n, y.data()) == IPT_OK;
 }
 
-int tuned_candidate(const Shape& s) {
+int tuned_candidate(const DeviceCtx* ctx, const Shape& s) {
     std::lock_guard<std::mutex> g(g_sel_mu);
-    auto it = g_tuned.find(std::make_tuple(s.esz, s.d1, s.d2, s.d3));
-    return it == g_tuned.end() ? -1 : it->second;
+    return tuned_locked(ctx, s);
 }
 
 bool candidate_ok(const DeviceCtx* ctx, const Shape& s, int k) { return candidate_feasible(ctx, s, k); }
@@ -244,6 +278,7 @@ int ipt_select_candidate(int elem_bytes, int64_t d1, int64_t d2, int64_t d3) {
 ipt_status ipt_tuning_clear(void) {
     std::lock_guard<std::mutex> g(g_sel_mu);
     g_tuned.clear();
+    g_tuned_class.clear();
     g_sel_cache.clear();
     return IPT_OK;
 }
@@ -254,10 +289,15 @@ ipt_status ipt_tuning_save(const char* path) {
     FILE* f = std::fopen(path, "w");
     if (!f) return IPT_ERR_IO;
     std::fprintf(f, "# ipt tuning table: elem_bytes d1 d2 d3 candidate\n");
+    std::fprintf(f, "# shape classes: class elem_bytes rows_bucket cols_bucket d3_bucket square "
+                    "candidate\n");
     for (const auto& kv : g_tuned)
         std::fprintf(f, "%d %lld %lld %lld %d\n", std::get<0>(kv.first),
                      (long long)std::get<1>(kv.first), (long long)std::get<2>(kv.first),
                      (long long)std::get<3>(kv.first), kv.second);
+    for (const auto& kv : g_tuned_class)
+        std::fprintf(f, "class %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
+                     std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first), kv.second);
     const bool ok = std::fclose(f) == 0;
     return ok ? IPT_OK : IPT_ERR_IO;
 }
@@ -267,10 +307,22 @@ ipt_status ipt_tuning_load(const char* path) {
     FILE* f = std::fopen(path, "r");
     if (!f) return IPT_ERR_IO;
     std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> table;
+    std::map<ShapeClass, int> classes;
     char line[256];
     bool bad = false;
     while (std::fgets(line, sizeof(line), f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
+        if (std::strncmp(line, "class ", 6) == 0) {
+            int esz, rb, cb, db, sq, k;
+            if (std::sscanf(line + 6, "%d %d %d %d %d %d", &esz, &rb, &cb, &db, &sq, &k) != 6 ||
+                (esz != 4 && esz != 8) || rb < 0 || cb < 0 || db < 0 || (sq != 0 && sq != 1) ||
+                k < 0 || k >= kNumCandidates) {
+                bad = true;
+                break;
+            }
+            classes[std::make_tuple(esz, rb, cb, db, sq)] = k;
+            continue;
+        }
         int esz, k;
         long long d1, d2, d3;
         if (std::sscanf(line, "%d %lld %lld %lld %d", &esz, &d1, &d2, &d3, &k) != 5 ||
@@ -284,6 +336,7 @@ ipt_status ipt_tuning_load(const char* path) {
     if (bad) return IPT_ERR_IO;
     std::lock_guard<std::mutex> g(g_sel_mu);
     for (const auto& kv : table) g_tuned[kv.first] = kv.second;
+    for (const auto& kv : classes) g_tuned_class[kv.first] = kv.second;
     g_sel_cache.clear();
     return IPT_OK;
 }
@@ -355,6 +408,7 @@ ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
     if (e == hipSuccess && st == IPT_OK && bestk >= 0) {
         std::lock_guard<std::mutex> g(g_sel_mu);
         g_tuned[std::make_tuple(s.esz, s.d1, s.d2, s.d3)] = bestk;
+        g_tuned_class[shape_class(s)] = bestk;
         g_sel_cache.clear();
     }
     if (a) (void)hipEventDestroy(a);

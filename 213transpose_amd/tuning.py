@@ -1,11 +1,11 @@
 """Autotuner sweeps, selector training and evaluation.
 
-    python 213transpose_amd/autotune.py sweep --count 22000 --seed 5 \
+    python 213transpose_amd/tuning.py sweep --count 22000 --seed 5 \
         --start 0 --stop 5000 --out tune_part0.npz
-    python 213transpose_amd/autotune.py merge --out data.npz part0.npz part1.npz ...
-    python 213transpose_amd/autotune.py train --data data.npz --heldout 2000 \
+    python 213transpose_amd/tuning.py merge --out data.npz part0.npz part1.npz ...
+    python 213transpose_amd/tuning.py train --data data.npz --heldout 2000 \
         --out 213transpose_amd/selector/weights.f32
-    python 213transpose_amd/autotune.py evaluate --data data.npz --heldout 2000 \
+    python 213transpose_amd/tuning.py evaluate --data data.npz --heldout 2000 \
         --weights 213transpose_amd/selector/weights.f32
 
 sweep: seeded shapes with m, n log-uniform in [2, 65536] and 4- or 8-byte

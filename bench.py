@@ -520,7 +520,7 @@ def selector_eval(ipt):
     if not os.path.exists(path) or not os.path.exists(ipt.SELECTOR_WEIGHTS):
         return None
     import numpy as np
-    at = importlib.import_module("213transpose_amd.autotune")
+    at = importlib.import_module("213transpose_amd.tuning")
     d = np.load(path)
     p = np.fromfile(ipt.SELECTOR_WEIGHTS, dtype=np.float32)
     res = at.evaluate_arrays(d["shapes"][-2000:], d["feats"][-2000:], d["times"][-2000:], p)

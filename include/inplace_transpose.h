@@ -210,6 +210,12 @@ typedef struct {
     int32_t block;        /* threads per workgroup (power of two, 64..1024) */
     int32_t strip;        /* column-strip width for LDS column passes (elements) */
     int32_t grid_mult;    /* workgroups per CU for grid-stride kernels */
+    int32_t chunk_min;    /* block-cycle path: chunk sizes (bytes) the plan may    */
+    int32_t chunk_max;    /*   pick from (default 512 .. 32768); its price model
+                             ranks the divisors T of the chunked side inside them */
+    int32_t tile;         /* square swap: tile side in elements (64 / 32 for 4- /
+                             8-byte elements by default; half that is the other
+                             compiled size) */
 } ipt_tune;
 
 /* Same semantics as ipt_transpose_213 with a forced variant and optional
@@ -295,7 +301,7 @@ ipt_status ipt_debug_arena_grow(int which, size_t bytes);
  * candidate with the lowest prediction, for problems up to 1 GiB of payload
  * (the domain of its training sweep).  Without a loaded selector, and above
  * that size, the plain heuristic decides. */
-#define IPT_NUM_CANDIDATES 9
+#define IPT_NUM_CANDIDATES 13
 #define IPT_MLP_IN 8
 #define IPT_MLP_HIDDEN 64
 /* Number of fp32 parameters: W1, b1, W2, b2, W3, b3 (row-major, out x in). */
@@ -319,10 +325,14 @@ ipt_status ipt_autotune(void* data, int elem_bytes, int64_t d1, int64_t d2, int6
                         float* times_us, int* best);
 
 /* Measured choices: every successful ipt_autotune records the fastest
- * candidate for its exact shape; IPT_VARIANT_AUTO uses a recorded choice
- * before the selector.  The table can be saved and loaded as text
- * ("elem_bytes d1 d2 d3 candidate" per line; IPT_ERR_IO on a bad file, which
- * then changes nothing). */
+ * candidate for its exact shape and for its shape class (element size,
+ * half-octave buckets of the rows d2 and columns d1, octave bucket of d3,
+ * square or not); IPT_VARIANT_AUTO and batches use a recorded choice before
+ * the selector, the exact shape's first, else the class's where that
+ * candidate can run the shape.  The table can be saved and loaded as text
+ * ("elem_bytes d1 d2 d3 candidate" and "class elem_bytes rows_bucket
+ * cols_bucket d3_bucket square candidate" lines; IPT_ERR_IO on a bad file,
+ * which then changes nothing). */
 ipt_status ipt_tuning_clear(void);
 ipt_status ipt_tuning_save(const char* path);
 ipt_status ipt_tuning_load(const char* path);

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DATA = os.path.join(ROOT, "213transpose_amd", "selector", "sweep.npz")
 WEIGHTS = os.path.join(ROOT, "213transpose_amd", "selector", "weights.f32")
 HELDOUT = 2000
-NOUT = 9  # candidates (IPT_NUM_CANDIDATES)
+NOUT = 13  # candidates (IPT_NUM_CANDIDATES)
 
 
 def unpack(p, nin=8, nh=64, nout=NOUT):
@@ -117,7 +117,7 @@ def test_trained_selector_beats_heuristic_on_heldout(ipt):
     """Held-out shapes of the autotuner's own sweep: the learned choice's
     measured time against exhaustive search and against the heuristic's."""
     import importlib
-    at = importlib.import_module("213transpose_amd.autotune")
+    at = importlib.import_module("213transpose_amd.tuning")
     d = np.load(DATA)
     p = np.fromfile(WEIGHTS, dtype=np.float32)
     res = at.evaluate_arrays(d["shapes"][-HELDOUT:], d["feats"][-HELDOUT:], d["times"][-HELDOUT:], p)
@@ -235,7 +235,7 @@ def test_gpu_training_on_sweep_beats_heuristic_heldout(ipt):
     search, and less total time than the heuristic's picks."""
     import importlib
     import time
-    at = importlib.import_module("213transpose_amd.autotune")
+    at = importlib.import_module("213transpose_amd.tuning")
     d = np.load(DATA)
     n = len(d["feats"]) - HELDOUT
     assert n >= 20000
@@ -252,3 +252,84 @@ def test_gpu_training_on_sweep_beats_heuristic_heldout(ipt):
     assert res["shapes"] >= 1900
     assert res["regret_mean"] < res["heuristic_regret_mean"]
     assert res["total_time_speedup_vs_heuristic"] > 1.0
+
+
+@pytest.mark.gpu
+def test_tuned_chunk_classes_beat_price_model(ipt, oracle):
+    """The autotuner's block-cycle chunk classes (candidates 9-11: the price
+    model ranks the divisors T of the chunked side inside 512 B .. 2 KB,
+    2 .. 8 KB and 8 .. 32 KB) against the price model's own choice over the
+    whole range (candidate 8), on shapes where the sweep found smaller chunks
+    faster: every block-cycle candidate is exact, the best chunk class is at
+    least 10 % faster than the price model's T on some of them, and a
+    neighbouring shape of the same class dispatches to the autotuner's
+    recorded choice."""
+    import torch
+    shapes = [(47628, 23, 1, 4), (1978, 276, 1, 4), (11919, 36, 1, 8), (14440, 68, 1, 4),
+              (150, 19932, 1, 4)]
+    ipt.tuning_clear()
+    try:
+        wins = []
+        for k, (d1, d2, d3, esz) in enumerate(shapes):
+            a = oracle.random_bits(d1 * d2 * d3, esz, 1500 + k)
+            exp = oracle.seq_tt_213(a, d1, d2, d3)
+            dt = np.int32 if esz == 4 else np.int64
+            for c in (8, 9, 10, 11):
+                v, tune = ipt.candidate(c)
+                t = torch.from_numpy(a.view(dt)).cuda()
+                try:
+                    ipt.transpose_variant(t, d1, d2, d3, v, tune=tune)
+                except ipt.TransposeError as e:
+                    assert e.status == 2  # no plan with chunks in this class
+                    continue
+                torch.cuda.synchronize()
+                assert np.array_equal(t.cpu().numpy().view(a.dtype), exp), (d1, d2, c)
+            t = torch.from_numpy(a.view(dt)).cuda()
+            times, best = ipt.autotune(t, d1, d2, d3, reps=10)
+            print(f"[chunk classes] {(d1, d2, d3, esz)}: price model {times[8]:.1f} us, "
+                  f"classes {[round(float(x), 1) for x in times[9:12]]}, best {best}", flush=True)
+            tuned = [x for x in times[9:12] if x > 0]
+            if times[8] > 0 and tuned and min(tuned) < 0.9 * times[8]:
+                wins.append((d1, d2, d3, esz, best))
+            recorded = (d1, d2, d3, esz, best)
+        assert wins, "no chunk class beat the price model's choice of T by 10 %"
+        # the measured choice (whichever candidate won) is recorded for the
+        # shape's class: a shape with the same half-octave buckets dispatches
+        # to it (when it can run there)
+        d1, d2, d3, esz, best = recorded
+        n1 = d1 - 4 if d1 > 4 else d1 + 4
+        v, tune = ipt.candidate(best)
+        a = oracle.random_bits(n1 * d2 * d3, esz, 1600)
+        t = torch.from_numpy(a.view(np.int32 if esz == 4 else np.int64)).cuda()
+        try:
+            ipt.transpose_variant(t, n1, d2, d3, v, tune=tune)
+            runnable = True
+        except ipt.TransposeError:
+            runnable = False
+        if runnable and int(np.floor(2 * np.log2(n1))) == int(np.floor(2 * np.log2(d1))):
+            assert ipt.select_candidate(esz, n1, d2, d3) == best
+    finally:
+        ipt.tuning_clear()
+
+
+def test_tuning_table_shape_classes_roundtrip(ipt, tmp_path):
+    """Class lines of the tuning table: loaded, used for every shape of the
+    class (half-octave buckets of rows and columns, octave bucket of d3,
+    square or not), saved back."""
+    path = tmp_path / "classes.txt"
+    # f32, rows 1000 (bucket floor(2 log2 1000) = 19), columns 3000 (23), d3 1 (0), not square
+    path.write_text("class 4 19 23 0 0 2\n")
+    ipt.tuning_clear()
+    try:
+        ipt.tuning_load(str(path))
+        assert ipt.select_candidate(4, 3000, 1000, 1) == 2
+        assert ipt.select_candidate(4, 2950, 1010, 1) == 2   # same buckets
+        out = tmp_path / "saved.txt"
+        ipt.tuning_save(str(out))
+        assert "class 4 19 23 0 0 2" in out.read_text()
+        bad = tmp_path / "bad.txt"
+        bad.write_text("class 4 19 23 0 0 99\n")
+        with pytest.raises(ipt.TransposeError):
+            ipt.tuning_load(str(bad))
+    finally:
+        ipt.tuning_clear()
