@@ -94,6 +94,9 @@ def parse():
                     help="one reference config shape (1, 2a, ..., 3d): calibration copies of "
                          "its bytes, then warmup + steps transposes with the default dispatch "
                          "(for profilers)")
+    ap.add_argument("--batch-routes", action="store_true",
+                    help="the uneven batch's time by route and by slice size: the workload's "
+                         "instances of each route run as a batch of their own")
     ap.add_argument("--configs", action="store_true",
                     help="time every reference config shape with the default dispatch (one JSON line)")
     ap.add_argument("--only", default=None,
@@ -323,6 +326,50 @@ def profile_batch_main(args, torch, ipt):
     print("profile run done", flush=True)
 
 
+def batch_routes_main(args, torch, ipt):
+    """The uneven batch broken down: the instances of each route
+    (ipt_batch_get_routes of the whole batch) as a batch of their own, the
+    cooperative ones also by slice size; graph runs, median of 5 blocks of
+    3.  One line per part, then a plain copy of the payload."""
+    buf, items, payload = uneven_items(torch, ipt)
+    shapes = [(d1, d2, d3, t.element_size()) for t, d1, d2, d3 in items]
+
+    def graph_ms(b):
+        b.run(graph=True)
+        torch.cuda.synchronize()
+        return time_calls(torch, lambda _: b.run(graph=True), None, 3)
+
+    def part(name, idx):
+        nb = sum(shapes[i][0] * shapes[i][1] * shapes[i][2] * shapes[i][3] for i in idx)
+        b = ipt.Batch([items[i] for i in idx])
+        ms = graph_ms(b)
+        b.check()
+        b.close()
+        print(json.dumps({"part": name, "instances": len(idx), "bytes": nb, "ms": round(ms, 4),
+                          "GBps": round(2 * nb / (ms * 1e-3) / 1e9, 1)}), flush=True)
+
+    b = ipt.Batch(items)
+    routes = b.routes().tolist()
+    b.close()
+    part("whole batch", list(range(len(items))))
+    for r in sorted(set(routes)):
+        idx = [i for i, x in enumerate(routes) if x == r]
+        if r == ipt.ROUTE_SKIP:
+            continue
+        part(ipt.ROUTE_NAMES[r], idx)
+        if r == ipt.ROUTE_GCS:
+            edges = [0, 1 << 20, 4 << 20, 16 << 20, 48 << 20, 1 << 62]
+            for lo, hi in zip(edges[:-1], edges[1:]):
+                sub = [i for i in idx if lo <= shapes[i][0] * shapes[i][1] * shapes[i][3] < hi]
+                if sub:
+                    part(f"gcs slices [{lo >> 20}, {hi >> 20}) MB", sub)
+    raw = buf.view(torch.uint8)[:payload // 16 * 16]
+    sink = torch.empty_like(raw)
+    ms = time_calls(torch, lambda _: ipt.copy(sink, raw), None, 3)
+    print(json.dumps({"part": "plain copy of the payload", "bytes": raw.numel(), "ms": round(ms, 4),
+                      "GBps": round(2 * raw.numel() / (ms * 1e-3) / 1e9, 1)}), flush=True)
+
+
 def fill_iota(torch, items):
     for t, d1, d2, d3 in items:
         t.copy_(torch.arange(t.numel(), dtype=t.dtype, device="cuda"))
@@ -549,6 +596,9 @@ def main():
         return
     if args.profile_config:
         profile_config_main(args, torch, ipt)
+        return
+    if args.batch_routes:
+        batch_routes_main(args, torch, ipt)
         return
     if args.configs:
         configs_main(args, torch, ipt)
