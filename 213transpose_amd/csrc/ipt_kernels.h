@@ -662,6 +662,8 @@ constexpr uint32_t kOutside = 0x80000000u;  // past every resource built here (<
 // the XCD's L2, so a launch ends with no dirty lines for the end-of-kernel
 // write-back and the next launch starts about 4 us sooner (measured on the
 // [512][384][64] f32 permute: 24.4 -> 23.0 us per back-to-back launch).
+// Launches over larger slices store write-back instead (policy 0), which
+// lets the L2 merge the partial lines that neighbouring tiles share.
 constexpr int kStoreDevice = 0x10;
 
 // Raw buffer resource over [base, base + bytes); word 3 = 0x00020000 is the
@@ -1002,20 +1004,21 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
 }
 
 // GCS kernel: slot s (numbered by xcd_slot) is member s % g of
-// slice s / g.
-template <typename T, int BLOCK, int JR>
+// slice s / g.  SP: the stores' cache policy (kStoreDevice or 0, write-back);
+// ord: the members' tile order.
+template <typename T, int BLOCK, int JR, int SP>
 __global__ void __launch_bounds__(BLOCK)
 k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g, uint32_t K,
        uint32_t KL, Div32 dntj, uint32_t ntiles, uint32_t* __restrict__ ctl,
-       uint32_t* __restrict__ cnt, uint32_t nxcd) {
+       uint32_t* __restrict__ cnt, uint32_t nxcd, TileOrder ord) {
     IPT_DYN_LDS(u32x4, lds);
     const uint64_t total = nslices * g;
     const uint64_t slice_elems = (uint64_t)R * C;
     for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd, g); slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
-        gcs_body<T, BLOCK, JR, kStoreDevice>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
-                            g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds);
+        gcs_body<T, BLOCK, JR, SP>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
+                                   g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds, ord);
         __syncthreads();  // LDS is reused by the next slot
     }
 }

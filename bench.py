@@ -115,7 +115,9 @@ def configs_main(args, torch, ipt):
     """Every config shape, iota input: one transpose checked against the
     gen_ans digest and undone; then, for the transpose and for its inverse
     apart, 3 warm-up calls and the median over 5 event-timed blocks of 10
-    back-to-back calls.  The work of an in-place permutation does not depend
+    back-to-back calls; the same for the out-of-place comparator
+    (ipt_transpose_oop into a second buffer, its output checked against the
+    same digest).  The work of an in-place permutation does not depend
     on the values, so repeated calls time the same work."""
     import numpy as np
     out = []
@@ -131,21 +133,30 @@ def configs_main(args, torch, ipt):
         ok = sha == golden_sha((d1, d2, d3, esz))
         ipt.transpose(t, d2, d1, d3)
 
-        def per_call(a, b):
+        def per_call(a, b, call=lambda a, b: ipt.transpose(t, a, b, d3)):
             for _ in range(3):
-                ipt.transpose(t, a, b, d3)
+                call(a, b)
             blocks = []
             for _ in range(5):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
                 for _ in range(10):
-                    ipt.transpose(t, a, b, d3)
+                    call(a, b)
                 e1.record()
                 e1.synchronize()
                 blocks.append(e0.elapsed_time(e1) / 10)
             return float(np.median(blocks))
         ms = per_call(d1, d2)
         ms_inv = per_call(d2, d1)
+        # the out-of-place comparator on the same shape, from the iota input
+        t.copy_(torch.arange(d1 * d2 * d3, dtype=dt, device="cuda"))
+        dst = torch.empty_like(t)
+        ipt.transpose_oop(dst, t, d1, d2, d3)
+        torch.cuda.synchronize()
+        oop_ok = hashlib.sha256(dst.cpu().numpy().tobytes()).hexdigest() == golden_sha(
+            (d1, d2, d3, esz))
+        ms_oop = per_call(d1, d2, lambda a, b: ipt.transpose_oop(dst, t, a, b, d3))
+        del dst
         ipt.device_check()
         nb = 2 * d1 * d2 * d3 * esz
         k = ipt.select_candidate(esz, d1, d2, d3)
@@ -155,7 +166,8 @@ def configs_main(args, torch, ipt):
                     "frac_of_hbm_peak": round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "inverse_variant": ipt.VARIANT_NAMES[ipt.candidate(
                         ipt.select_candidate(esz, d2, d1, d3))[0]],
-                    "inverse_GBps": round(nb / (ms_inv * 1e-3) / 1e9, 1)})
+                    "inverse_GBps": round(nb / (ms_inv * 1e-3) / 1e9, 1),
+                    "oop_GBps": round(nb / (ms_oop * 1e-3) / 1e9, 1), "oop_verified": oop_ok})
         print(f"[config {name}] {out[-1]}", file=sys.stderr, flush=True)
         del t
         torch.cuda.empty_cache()
