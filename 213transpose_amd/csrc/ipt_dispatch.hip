@@ -427,18 +427,24 @@ static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t
     const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, 2, true);
     if (!gg.ok) return IPT_ERR_UNSUPPORTED;
     Arena& a = t.sync ? *t.sync : ctx->sync;
-    // Slices of 1 MiB or more: banded tile ownership and write-back stores
-    // (the L2 merges the partial lines of neighbouring tiles before they
-    // leave it).  Smaller slices keep the strided order and device-scope
-    // stores, whose launches end with no dirty lines to write back.
-    // Measured on MI355X, ms per launch, device-scope strided -> write-back
-    // banded: f32 97 x 20011 x 200 0.971 -> 0.813, f32 513 x 769 x 300
-    // 0.280 -> 0.244, f32 1024 x 2048 x 128 0.485 -> 0.430, f64 2048 x 2048
-    // 0.0314 -> 0.0253, f64 256 x 1000 x 17 0.0155 -> 0.0144, f32 4096 x 1000
-    // x 60 0.519 -> 0.477, f64 2543 x 3279 x 16 0.620 -> 0.626; 786 KB slices
-    // lost (f32 512 x 384 x 64 0.0188 -> 0.0201, x 256 0.0670 -> 0.0726).
-    // Banded order with device-scope stores was slower than either.
-    const bool wide = (uint64_t)s.d1 * (uint64_t)s.d2 * sizeof(T) >= ((uint64_t)1 << 20);
+    // Banded tile ownership and write-back stores (the L2 merges the partial
+    // lines of neighbouring tiles before they leave it) where they pay:
+    // slices of 1 MiB or more in packed launches (groups take several
+    // rounds, so each round's dirty lines are written back while later
+    // rounds run) or in single-round launches whose members hold 56 tiles or
+    // more.  Elsewhere the strided order and device-scope stores, whose
+    // launches end with no dirty lines to write back: short single-round
+    // launches lose more to that end-of-kernel write-back than the merging
+    // saves.  Measured on MI355X, us per call with the default dispatch,
+    // device scope strided -> write-back banded: packed f32 513 x 769 x 300
+    // 268 -> 224, 1000 x 3000 x 40 266 -> 254; single round, K >= 56: f64
+    // 1000 x 256 x 17 16.0 -> 14.2, f32 3001 x 2999 x 2 34.2 -> 32.5; single
+    // round, K <= 48: f32 3000 x 1000 (K = 12) 16.6 -> 16.7, 1500 x 6000 (35)
+    // 26.5 -> 26.9, 700 x 5000 x 3 (41) 22.9 -> 24.2; 786 KB slices: f32 512
+    // x 384 x 64 17.8 -> 19.8.  Banded order with device-scope stores, and
+    // non-temporal stores, were slower than either.
+    const bool wide = (uint64_t)s.d1 * (uint64_t)s.d2 * sizeof(T) >= ((uint64_t)1 << 20) &&
+                      (gg.grid != 0 || gg.K >= 56);
     const TileOrder ord = wide ? banded_order(gg.K, gg.ntiles / gg.ntj, gg.ntj) : TileOrder{};
     const int cls = gcs_jr_class(sizeof(T), gg.K, gg.KL, kGcsBlock / 64);
     if (wide) {
