@@ -356,7 +356,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             // members own blocks of adjacent tiles: uneven 4096-matrix batch
             // 4.48 -> 4.29 ms (write traffic 1.13 -> 1.07x, most rows not
             // 128-byte aligned)
-            I.ord = banded_order(gg.K, gg.ntiles / gg.ntj, gg.ntj);
+            I.ord = banded_order(kBatchBlock / 64, gg.ntiles / gg.ntj, gg.ntj);
             I.bar = bar_words_total;
             bar_words_total += bar_words(gg.g) * (uint64_t)s.d3;
             lds_need = std::max(lds_need, gg.lds);

@@ -445,7 +445,7 @@ static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t
     // non-temporal stores, were slower than either.
     const bool wide = (uint64_t)s.d1 * (uint64_t)s.d2 * sizeof(T) >= ((uint64_t)1 << 20) &&
                       (gg.grid != 0 || gg.K >= 56);
-    const TileOrder ord = wide ? banded_order(gg.K, gg.ntiles / gg.ntj, gg.ntj) : TileOrder{};
+    const TileOrder ord = wide ? banded_order(kGcsBlock / 64, gg.ntiles / gg.ntj, gg.ntj) : TileOrder{};
     const int cls = gcs_jr_class(sizeof(T), gg.K, gg.KL, kGcsBlock / 64);
     if (wide) {
         switch (cls) {

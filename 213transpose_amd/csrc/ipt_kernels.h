@@ -886,7 +886,7 @@ inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw) {
 // walked band by band, a band being bh tile rows (the last band may be
 // shorter) walked column by column: position p = b * bh * ntj + tj * h + r is
 // tile (b * bh + r, tj), h the band's height; member m owns positions
-// m * K .. m * K + nk - 1, about a bh x (K / bh) block of adjacent tiles.
+// m * K .. m * K + nk - 1, a block of adjacent tiles about bh tall.
 // Adjacent tiles share the 128-byte lines that a row not aligned to 128
 // bytes splits, on the input side (along a tile row) and on the output side
 // (along a tile column); in a block those stay with one member, instead of
@@ -900,13 +900,16 @@ struct TileOrder {
     Div32 dlast{};      // the last, shorter band's height (unused when none)
 };
 
-// Banded order for a group whose members hold up to K tiles of a slice with
-// nti x ntj tiles: bands about sqrt(K) tiles tall, so a member's block is
-// roughly square (host side).
-inline TileOrder banded_order(uint32_t K, uint32_t nti, uint32_t ntj) {
+// Banded order over a slice of nti x ntj tiles for members of nw waves:
+// bands nw tiles tall.  Wave w of a member takes its tiles k = w + j * nw
+// (gcs_body), which in a band of nw rows are one tile row, consecutive
+// columns: the line two horizontally adjacent tiles share is then requested
+// by back-to-back loads of one wave and fetched once, where loads from
+// different waves fetch it twice (read probe on misaligned rows: 1.03-1.06x
+// vs 1.12-1.22x the bytes).  Host side.
+inline TileOrder banded_order(uint32_t nw, uint32_t nti, uint32_t ntj) {
     TileOrder o;
-    uint32_t bh = 1;
-    while ((bh + 1) * (bh + 1) <= K) bh++;
+    const uint32_t bh = nw;
     o.bh = bh < nti ? bh : nti;
     o.full = nti / o.bh;
     o.dband = Div32::make(o.bh * ntj);
