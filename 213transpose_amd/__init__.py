@@ -186,6 +186,8 @@ def lib():
                                        ctypes.POINTER(_i64)]
     L.ipt_debug_arena_grow.restype = _int
     L.ipt_debug_arena_grow.argtypes = [_int, ctypes.c_size_t]
+    L.ipt_debug_chunk_moves.restype = _int
+    L.ipt_debug_chunk_moves.argtypes = [_i64, _i64, _i64, _int, _vp, _vp, _vp, _vp, _vp, _i64]
     L.ipt_transpose_oop.restype = _int
     L.ipt_transpose_oop.argtypes = [_vp, _vp, _int, _i64, _i64, _i64, _vp]
     L.ipt_memory_kind.restype = _int
@@ -648,6 +650,25 @@ def arena_info(which):
 def arena_grow(which, nbytes):
     """Force a per-device arena to grow to at least nbytes (test hook)."""
     _check(lib().ipt_debug_arena_grow(which, nbytes), "ipt_debug_arena_grow")
+
+
+def chunk_moves(rc, cc, ell, device=True):
+    """The chunk step's move lists of the rc x cc chunk transpose with
+    segments of at most ell moves, built by the device builder (or the host
+    walk): (items (n, 2) of (first move, moves), msrc, mdst, hpos), uint32
+    numpy arrays (test hook)."""
+    import numpy as np
+    cap = rc * cc
+    counts = np.zeros(3, dtype=np.int64)
+    items = np.zeros((max(cap, 1), 2), dtype=np.uint32)
+    msrc = np.zeros(max(cap, 1), dtype=np.uint32)
+    mdst = np.zeros(max(cap, 1), dtype=np.uint32)
+    hpos = np.zeros(max(cap, 1), dtype=np.uint32)
+    _check(lib().ipt_debug_chunk_moves(rc, cc, ell, 1 if device else 0, counts.ctypes.data,
+                                       items.ctypes.data, msrc.ctypes.data, mdst.ctypes.data,
+                                       hpos.ctypes.data, cap), "ipt_debug_chunk_moves")
+    ni, nm, nh = (int(c) for c in counts)
+    return items[:ni].copy(), msrc[:nm].copy(), mdst[:nm].copy(), hpos[:nh].copy()
 
 
 def version():

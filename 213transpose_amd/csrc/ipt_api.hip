@@ -656,6 +656,37 @@ ipt_status ipt_debug_set_watchdog(uint32_t polls, uint32_t extra) {
     return IPT_OK;
 }
 
+ipt_status ipt_debug_chunk_moves(int64_t rc, int64_t cc, int64_t ell, int builder,
+                                 int64_t counts[3], uint32_t* items, uint32_t* msrc,
+                                 uint32_t* mdst, uint32_t* hpos, int64_t cap) {
+    if (rc < 1 || cc < 1 || rc * cc >= kBcMaxChunks || ell < 1 || ell > kBcMaxSeg ||
+        (builder != 0 && builder != 1) || !counts || cap < 0)
+        return IPT_ERR_INVALID_ARG;
+    DeviceCtx* ctx;
+    ipt_status st = get_ctx(&ctx);
+    if (st != IPT_OK) return st;
+    ChunkMoves m;
+    if ((st = chunk_moves_debug(rc, cc, ell, builder == 1, &m)) != IPT_OK) return st;
+    counts[0] = m.nitems;
+    counts[1] = m.nmoves;
+    counts[2] = m.nheads;
+    if ((int64_t)m.nitems > cap || (int64_t)m.nmoves > cap || (int64_t)m.nheads > cap) {
+        chunk_moves_free(&m);
+        return IPT_ERR_TOO_LARGE;
+    }
+    hipError_t e = hipSuccess;
+    if (m.nitems) e = hipMemcpy(items, m.items, (size_t)m.nitems * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && m.nmoves) e = hipMemcpy(msrc, m.msrc, (size_t)m.nmoves * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && m.nmoves) e = hipMemcpy(mdst, m.mdst, (size_t)m.nmoves * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && m.nheads) e = hipMemcpy(hpos, m.hpos, (size_t)m.nheads * 4, hipMemcpyDeviceToHost);
+    chunk_moves_free(&m);
+    if (e != hipSuccess) {
+        set_error("hipMemcpy", e);
+        return IPT_ERR_HIP;
+    }
+    return IPT_OK;
+}
+
 ipt_status ipt_debug_arena_info(int which, void** ptr, size_t* bytes, int64_t* retired) {
     if ((which != 0 && which != 1) || !ptr || !bytes || !retired) return IPT_ERR_INVALID_ARG;
     DeviceCtx* ctx;

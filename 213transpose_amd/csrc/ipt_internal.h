@@ -232,12 +232,16 @@ struct ChunkMoves {
     uint32_t* msrc = nullptr;  // source position, or a flagged head slot
     uint32_t* mdst = nullptr;  // destination position
     uint32_t* hpos = nullptr;  // position of each saved head
-    uint32_t nitems = 0, nheads = 0;
+    uint32_t nitems = 0, nheads = 0, nmoves = 0;
     size_t bytes = 0;  // device bytes held
 };
-// Builds and uploads the move lists of plan p; the caller owns them and
-// frees them with chunk_moves_free.  Allocates: never under capture.
+// Builds the move lists of plan p on the device (returns when they are
+// complete); the caller owns them and frees them with chunk_moves_free.
+// Allocates: never under capture.
 ipt_status chunk_moves_create(const BlockCyclePlan& p, ChunkMoves* out);
+// The lists of the rc x cc chunk transpose with segments of ell, built on
+// the device (as chunk_moves_create) or by the host walk.
+ipt_status chunk_moves_debug(int64_t rc, int64_t cc, int64_t ell, bool device, ChunkMoves* out);
 void chunk_moves_free(ChunkMoves* m);
 // Raises the chunk kernel's dynamic-LDS limit on this device (once).
 ipt_status chunk_kernels_setup(const DeviceCtx* ctx);

@@ -291,6 +291,18 @@ ipt_status ipt_debug_set_watchdog(uint32_t polls, uint32_t extra);
 ipt_status ipt_debug_arena_info(int which, void** ptr, size_t* bytes, int64_t* retired);
 ipt_status ipt_debug_arena_grow(int which, size_t bytes);
 
+/* Test hook for the chunk step's move lists (the cycles of the rc x cc
+ * chunk-matrix transpose, src/permute.cu:18-49 in the reference, which walks
+ * them on the host): builds them with segments of at most `ell` moves
+ * (1..128; rc * cc < 2^23) by the library's device builder (builder 1) or
+ * by the host walk (builder 0), and copies them out: items as (first move,
+ * moves) pairs, the moves' sources (a saved head's slot with the top bit
+ * set) and destinations, the heads' positions.  counts = (items, moves,
+ * heads); IPT_ERR_TOO_LARGE, with counts set, when one exceeds cap. */
+ipt_status ipt_debug_chunk_moves(int64_t rc, int64_t cc, int64_t ell, int builder,
+                                 int64_t counts[3], uint32_t* items, uint32_t* msrc,
+                                 uint32_t* mdst, uint32_t* hpos, int64_t cap);
+
 /* ---- autotuner and learned selector ----------------------------------- */
 /* No counterpart in the reference: its tier thresholds are magic numbers
  * (src/shuffle.cu:15-50, src/rotate.cu:241-258).  Here IPT_NUM_CANDIDATES
