@@ -150,7 +150,10 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
     if (slice_bytes <= lds_cap) return Route::Slice;
     if (s.d1 == s.d2) return Route::Swap;  // faster than the cooperative route on squares
     if (gcs_ok) return Route::Gcs;
-    *big_variant = s.d1 >= s.d2 ? IPT_VARIANT_C2R : IPT_VARIANT_R2C;
+    // Too large for one group of the batch kernel: the block-cycle path with
+    // its blocks folded into the batch (two sweeps) when a plan fits, which
+    // the caller tries first, else C2R / R2C as separate launches.
+    *big_variant = IPT_VARIANT_BLOCK_CYCLE;
     (void)t;
     return Route::Big;
 }
@@ -384,14 +387,24 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     // one chunk-matrix shape).  false = the instance cannot go this way.
     std::map<std::tuple<int64_t, int64_t, int64_t>, size_t> moves_of;
     size_t head_bytes = 0;
+    // Plans are made for the batch kernel's register-tile class; a chunk-size
+    // range (a tuned or learned chunk class) that leaves no block this kernel
+    // runs falls back to the whole range before the instance leaves the batch.
     auto add_block_cycle = [&](void* data, const Shape& s, const Tune& t, ipt_status* err) -> bool {
         *err = IPT_OK;
-        const BlockCyclePlan bp =
-            block_cycle_for(block_cycle_plan(ctx, s, t.chunk_min, t.chunk_max), data);
-        if (!bp.ok) return false;
-        int bv = IPT_VARIANT_AUTO;
-        const Route rb = plan_instance(ctx, bp.blk, bp.block_variant, lds_cap, grid, b->tune, &bv);
-        if (rb == Route::Big || rb == Route::Skip) return false;
+        BlockCyclePlan bp;
+        Route rb = Route::Big;
+        for (int pass = 0; pass < 2 && rb == Route::Big; pass++) {
+            if (pass == 1 && t.chunk_min <= 0 && t.chunk_max <= 0) break;
+            bp = block_cycle_for(block_cycle_plan(ctx, s, pass ? 0 : t.chunk_min,
+                                                  pass ? 0 : t.chunk_max, kBatchJrClass),
+                                 data);
+            if (!bp.ok) continue;
+            int bv = IPT_VARIANT_AUTO;
+            rb = plan_instance(ctx, bp.blk, bp.block_variant, lds_cap, grid, b->tune, &bv);
+            if (rb == Route::Skip) rb = Route::Big;
+        }
+        if (rb == Route::Big) return false;
         const auto key = std::make_tuple(bp.rc, bp.cc, bp.ell);
         auto it = moves_of.find(key);
         if (it == moves_of.end()) {

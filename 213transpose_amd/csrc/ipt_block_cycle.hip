@@ -168,10 +168,10 @@ k_chunk_cycle(V* __restrict__ data, const V* __restrict__ head, const uint2* __r
 //               side is narrower than the wave tile, whole-slice scalar
 //               form ~0.5).
 // The cheapest wins; ties go to the smaller chunk (smaller blocks).
-static int block_kernel(const DeviceCtx* ctx, const Shape& blk, double* eff) {
+static int block_kernel(const DeviceCtx* ctx, const Shape& blk, int jr_cls, double* eff) {
     const int64_t bytes = blk.d1 * blk.d2 * blk.esz;
     if (blk.d1 == blk.d2) { *eff = 1.0; return IPT_VARIANT_SQUARE_SWAP; }
-    const GcsGeom gg = gcs_geom(ctx, blk, lds_max_of(ctx), true, 2, true);
+    const GcsGeom gg = gcs_geom(ctx, blk, lds_max_of(ctx), true, jr_cls, true);
     const bool slice = bytes <= kBcLdsBlockBytes && variant_feasible(ctx, blk, IPT_VARIANT_SLICE_LDS);
     if (slice && (bytes % 16) == 0) { *eff = 0.9; return IPT_VARIANT_SLICE_LDS; }
     if (gg.ok) {
@@ -188,7 +188,7 @@ static int block_kernel(const DeviceCtx* ctx, const Shape& blk, double* eff) {
 }
 
 static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, int64_t cmin,
-                             int64_t cmax, BlockCyclePlan* best, double* best_cost) {
+                             int64_t cmax, int jr_cls, BlockCyclePlan* best, double* best_cost) {
     const int64_t esz = s.esz;
     const int64_t L = cols ? s.d1 : s.d2, other = cols ? s.d2 : s.d1;
     const int64_t tmax = std::min<int64_t>(L / 2, cmax / esz);
@@ -200,7 +200,7 @@ static void plan_orientation(const DeviceCtx* ctx, const Shape& s, bool cols, in
         const Shape blk = cols ? make_shape(s.esz, T, other, s.d3 * nb)
                                : make_shape(s.esz, other, T, s.d3 * nb);
         double eff;
-        const int v = block_kernel(ctx, blk, &eff);
+        const int v = block_kernel(ctx, blk, jr_cls, &eff);
         if (v == IPT_VARIANT_AUTO) continue;
         const int64_t chunk = T * esz;
         BlockCyclePlan c{};
@@ -251,15 +251,15 @@ BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data) {
 }
 
 BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s, int64_t chunk_min,
-                                int64_t chunk_max) {
+                                int64_t chunk_max, int jr_cls) {
     BlockCyclePlan p{};
     if (s.d1 == s.d2 || s.d1 < 2 || s.d2 < 2) return p;
     const int64_t cmin = std::max<int64_t>(chunk_min > 0 ? chunk_min : kBcMinChunk, kBcMinChunk);
     const int64_t cmax = std::min<int64_t>(chunk_max > 0 ? chunk_max : kBcMaxChunk, kBcMaxChunk);
     if (cmin > cmax) return p;
     double cost = 0;
-    plan_orientation(ctx, s, true, cmin, cmax, &p, &cost);
-    plan_orientation(ctx, s, false, cmin, cmax, &p, &cost);
+    plan_orientation(ctx, s, true, cmin, cmax, jr_cls, &p, &cost);
+    plan_orientation(ctx, s, false, cmin, cmax, jr_cls, &p, &cost);
     return p;
 }
 

@@ -206,9 +206,11 @@ struct BlockCyclePlan {
     int block_variant = 0;     // its one-sweep kernel
 };
 // The cheapest plan with chunks of chunk_min .. chunk_max bytes (0: the
-// defaults kBcMinChunk .. kBcMaxChunk).
+// defaults kBcMinChunk .. kBcMaxChunk) whose blocks a one-sweep kernel runs,
+// the cooperative kernel with register tiles of class jr_cls at most (the
+// persistent batch kernel stops at kBatchJrClass).
 BlockCyclePlan block_cycle_plan(const DeviceCtx* ctx, const Shape& s, int64_t chunk_min = 0,
-                                int64_t chunk_max = 0);
+                                int64_t chunk_max = 0, int jr_cls = 2);
 // The plan for data at `data`: vectors narrowed to the data's alignment.
 BlockCyclePlan block_cycle_for(const BlockCyclePlan& p, const void* data);
 // Parts and segment length of a plan whose T, blk.esz and vb are set.

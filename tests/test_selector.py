@@ -108,7 +108,7 @@ def test_selector_load_rejects_non_finite(ipt):
     p[5] = np.nan
     with pytest.raises(ipt.TransposeError):
         ipt.selector_load(p)
-    ipt.selector_load(None)
+    ipt.load_default_selector()  # later tests in this process see the shipped weights
 
 
 @pytest.mark.skipif(not (os.path.exists(DATA) and os.path.exists(WEIGHTS)),
@@ -181,7 +181,7 @@ def test_learned_dispatch_is_bit_exact(ipt, oracle):
             assert np.array_equal(t.cpu().numpy().view(a.dtype), oracle.seq_tt_213(a, d1, d2, d3))
         print("[selector] candidates used", sorted(used))
     finally:
-        ipt.selector_load(None)
+        ipt.load_default_selector()
 
 
 def test_tuning_table_roundtrip(ipt, tmp_path):
