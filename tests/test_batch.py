@@ -125,6 +125,34 @@ def test_batch_block_cycle_instances(ipt, oracle):
 
 
 @pytest.mark.gpu
+def test_batch_heuristic_folds_oversized_instances(ipt, oracle):
+    """With the selector off, an instance too large for one cooperative group
+    of the persistent kernel (f64 3751 x 3723, 112 MB; f64 3970 x 2975)
+    takes the block-cycle route with its blocks in the batch, not a separate
+    C2R / R2C launch, beside ordinary instances; every instance exact after an
+    eager run and a graph replay."""
+    shapes = [(3751, 3723, 1, 8), (3970, 2975, 1, 8), (512, 384, 2, 4), (33, 17, 4, 8)]
+    buf, items, host = make_instances(oracle, shapes, 1300)
+    ipt.selector_load(None)
+    try:
+        b = ipt.Batch(items)
+    finally:
+        ipt.load_default_selector()
+    routes = b.routes()
+    print("[batch heuristic oversized]", b.info(), routes, flush=True)
+    assert list(routes[:2]) == [ipt.ROUTE_BLOCK_CYCLE] * 2
+    assert b.info()["n_big"] == 0
+    b.run(graph=False)
+    torch.cuda.synchronize()
+    check(oracle, items, host, 1)
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, items, host, 2)
+    b.close()
+
+
+@pytest.mark.gpu
 def test_tiny_batches(ipt, oracle):
     """Batches with fewer work items than twice the XCD count run one shared
     queue (and a grid of a few workgroups)."""
