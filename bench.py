@@ -43,6 +43,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
                                                     against its gen_ans digest
        python bench.py --profile-batch | --profile-config NAME | --profile
                                                     (sequences for rocprofv3)
+       python bench.py --batch-routes | --move-lists   (breakdowns)
 """
 import argparse
 import concurrent.futures
@@ -97,6 +98,9 @@ def parse():
     ap.add_argument("--batch-routes", action="store_true",
                     help="the uneven batch's time by route and by slice size: the workload's "
                          "instances of each route run as a batch of their own")
+    ap.add_argument("--move-lists", action="store_true",
+                    help="the block-cycle chunk step's move lists at the plan limit (2^23 chunk "
+                         "positions): the device builder against the host walk")
     ap.add_argument("--configs", action="store_true",
                     help="time every reference config shape with the default dispatch (one JSON line)")
     ap.add_argument("--only", default=None,
@@ -336,6 +340,34 @@ def profile_batch_main(args, torch, ipt):
     b.check()
     b.close()
     print("profile run done", flush=True)
+
+
+MOVE_LISTS = [(32, 262143, 16), (7, 1198371, 64), (2896, 2896, 128), (32, 8192, 16)]
+
+
+def move_lists_main(args, torch, ipt):
+    """Build the chunk step's move lists of each (rows, cols, segment) chunk
+    matrix with the device builder and with the host walk (ipt_debug_chunk_moves:
+    build, copy the lists to the host, free), median of 3 after one warm-up
+    build of each; one JSON line."""
+    import numpy as np
+    out = []
+    ipt.chunk_moves(32, 512, 16)
+    ipt.chunk_moves(32, 512, 16, device=False)
+    for rc, cc, ell in MOVE_LISTS:
+        row = {"rows": rc, "cols": cc, "positions": rc * cc, "segment": ell}
+        for name, dev in (("device_ms", True), ("host_ms", False)):
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                lists = ipt.chunk_moves(rc, cc, ell, device=dev)
+                ts.append((time.perf_counter() - t0) * 1e3)
+            row[name] = round(float(np.median(ts)), 2)
+            row["moves"], row["heads"] = len(lists[1]), len(lists[3])
+        print(f"[move lists] {row}", file=sys.stderr, flush=True)
+        out.append(row)
+    print(json.dumps({"move_lists": out, "timing": "wall clock per build including the "
+                      "copy of the lists to the host, median of 3"}))
 
 
 def batch_routes_main(args, torch, ipt):
@@ -611,6 +643,9 @@ def main():
         return
     if args.batch_routes:
         batch_routes_main(args, torch, ipt)
+        return
+    if args.move_lists:
+        move_lists_main(args, torch, ipt)
         return
     if args.configs:
         configs_main(args, torch, ipt)
