@@ -387,9 +387,11 @@ def copy(dst, src, stream=None):
 
 
 # ipt_batch_route: how a batch runs each item.
-ROUTE_SKIP, ROUTE_SLICE, ROUTE_GCS, ROUTE_SWAP, ROUTE_BLOCK_CYCLE, ROUTE_SEPARATE = range(6)
+(ROUTE_SKIP, ROUTE_SLICE, ROUTE_GCS, ROUTE_SWAP, ROUTE_BLOCK_CYCLE, ROUTE_SEPARATE,
+ ROUTE_GCS_LAUNCH) = range(7)
 ROUTE_NAMES = {ROUTE_SKIP: "skip", ROUTE_SLICE: "slice", ROUTE_GCS: "gcs", ROUTE_SWAP: "swap",
-               ROUTE_BLOCK_CYCLE: "block_cycle", ROUTE_SEPARATE: "separate"}
+               ROUTE_BLOCK_CYCLE: "block_cycle", ROUTE_SEPARATE: "separate",
+               ROUTE_GCS_LAUNCH: "gcs_launch"}
 
 
 def transpose_oop(dst, src, d1, d2, d3=1, stream=None):

@@ -101,6 +101,10 @@ struct ipt_batch_s {
         Tune tune;
     };
     std::vector<Big> big;  // instances run by the per-call launchers
+    // Barrier state of the separate cooperative launches among `big`: fixed
+    // (allocated and zeroed when the batch is planned, never grown, so a
+    // captured run needs no allocation), wired to this batch's watchdog word.
+    Arena big_sync;
     // Chunk steps of block-cycle instances (their blocks are k_batch items).
     struct Chunk {
         void* data;
@@ -140,7 +144,10 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
     const bool gcs_ok = gg.ok && (int64_t)gg.g <= grid && gg.lds <= (size_t)lds_cap;
     switch (forced) {
     case IPT_VARIANT_SLICE_LDS: return slice_bytes <= lds_cap ? Route::Slice : Route::Big;
-    case IPT_VARIANT_GCS: return gcs_ok ? Route::Gcs : Route::Big;
+    case IPT_VARIANT_GCS:
+        if (gcs_ok) return Route::Gcs;
+        *big_variant = IPT_VARIANT_GCS;  // a separate launch, if one runs it
+        return Route::Big;
     case IPT_VARIANT_SQUARE_SWAP: return s.d1 == s.d2 ? Route::Swap : Route::Big;
     case IPT_VARIANT_C2R:
     case IPT_VARIANT_R2C:
@@ -150,10 +157,15 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
     if (slice_bytes <= lds_cap) return Route::Slice;
     if (s.d1 == s.d2) return Route::Swap;  // faster than the cooperative route on squares
     if (gcs_ok) return Route::Gcs;
-    // Too large for one group of the batch kernel: the block-cycle path with
-    // its blocks folded into the batch (two sweeps) when a plan fits, which
-    // the caller tries first, else C2R / R2C as separate launches.
-    *big_variant = IPT_VARIANT_BLOCK_CYCLE;
+    // Too large for one group of the batch kernel: one separate cooperative
+    // launch (one sweep, the per-call kernel's larger register tiles) where
+    // that fits; else the block-cycle path with its blocks folded into the
+    // batch (two sweeps) when a plan fits, which the caller tries next; else
+    // C2R / R2C as separate launches.
+    // (uneven 4096-matrix batch: three f64 instances of 88-96 MB as separate
+    // launches instead of folded block-cycle, graph 4.23 -> 4.08 ms)
+    *big_variant = gcs_geom(ctx, s, lds_max_of(ctx), true, 2, true).ok ? IPT_VARIANT_GCS
+                                                                       : IPT_VARIANT_BLOCK_CYCLE;
     (void)t;
     return Route::Big;
 }
@@ -162,9 +174,9 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
 // autotuner recorded one; else, with predictions y, the fastest predicted
 // route of the persistent kernel (whole slice, cooperative group, square
 // swap) when any fits, and only otherwise the fastest predicted candidate
-// with separate launches (block cycle, C2R, R2C: such launches cost more
-// inside a batch than their standalone timings show).  -1 = let
-// plan_instance's heuristic decide.
+// with separate launches (one cooperative launch of the per-call kernel,
+// block cycle, C2R, R2C: such launches cost more inside a batch than their
+// standalone timings show).  -1 = let plan_instance's heuristic decide.
 int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t lds_cap, int grid) {
     const int tuned = tuned_candidate(ctx, s);
     if (tuned >= 0) return tuned;
@@ -177,7 +189,7 @@ int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t l
     for (int k = 0; k < 4; k++)
         if (route_ok[k] && (best < 0 || y[k] < y[best])) best = k;
     if (best >= 0) return best;
-    for (int k = 4; k < kNumCandidates; k++)
+    for (int k = 2; k < kNumCandidates; k++)  // 2, 3: one separate cooperative launch
         if (candidate_ok(ctx, s, k) && (best < 0 || y[k] < y[best])) best = k;
     return best;
 }
@@ -226,6 +238,7 @@ void release(ipt_batch* b) {
     if (b->d_items) (void)hipFree(b->d_items);
     if (b->d_seg) (void)hipFree(b->d_seg);
     if (b->d_sync) (void)hipFree(b->d_sync);
+    if (b->big_sync.ptr) (void)hipFree(b->big_sync.ptr);
     for (auto& m : b->moves) chunk_moves_free(&m);
     if (b->d_head) (void)hipFree(b->d_head);
     if (b->wd_host) (void)hipHostFree(b->wd_host);
@@ -241,6 +254,9 @@ ipt_status reset_batch_sync(ipt_batch* b) {
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemset(b->d_sync, 0, (size_t)b->sync_words * sizeof(uint32_t));
     if (e == hipSuccess && b->wd_addr) e = ctl_init(b->d_sync, b->wd_addr);
+    if (e == hipSuccess && b->big_sync.ptr) e = hipMemset(b->big_sync.ptr, 0, b->big_sync.bytes);
+    if (e == hipSuccess && b->big_sync.ptr && b->wd_addr)
+        e = ctl_init((uint32_t*)b->big_sync.ptr, b->wd_addr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (b->wd_host) __atomic_store_n(b->wd_host, 0u, __ATOMIC_RELEASE);
     if (e != hipSuccess) {
@@ -387,6 +403,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     // one chunk-matrix shape).  false = the instance cannot go this way.
     std::map<std::tuple<int64_t, int64_t, int64_t>, size_t> moves_of;
     size_t head_bytes = 0;
+    size_t big_sync_bytes = 0;  // barrier state of the separate cooperative launches
     // Plans are made for the batch kernel's register-tile class; a chunk-size
     // range (a tuned or learned chunk class) that leaves no block this kernel
     // runs falls back to the whole range before the instance leaves the batch.
@@ -440,7 +457,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             if (k >= 0) {
                 forced = kCandidates[k].variant;
                 if (forced == IPT_VARIANT_C2R || forced == IPT_VARIANT_R2C ||
-                    forced == IPT_VARIANT_BLOCK_CYCLE)
+                    forced == IPT_VARIANT_BLOCK_CYCLE || forced == IPT_VARIANT_GCS)
                     big_tune = resolve_tune(ctx, &kCandidates[k].tune);
             }
         }
@@ -480,13 +497,22 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             release(b);
             return IPT_ERR_UNSUPPORTED;  // scratch tiers are not used inside a batch
         }
-        if (big_variant == IPT_VARIANT_GCS || !variant_feasible(ctx, s, big_variant)) {
+        if (!variant_feasible(ctx, s, big_variant)) {
             release(b);
             return IPT_ERR_UNSUPPORTED;
         }
+        if (big_variant == IPT_VARIANT_GCS) {
+            const GcsGeom gg = gcs_geom(ctx, s, big_tune.lds_cap, true, 2, true);
+            if (!gg.ok) {
+                release(b);
+                return IPT_ERR_UNSUPPORTED;
+            }
+            big_sync_bytes = std::max<size_t>(
+                big_sync_bytes, (size_t)(32 + bar_words(gg.g) * (uint64_t)s.d3) * sizeof(uint32_t));
+        }
         b->big.push_back({d.data, s, big_variant, big_tune});
         b->info.n_big++;
-        b->routes.push_back(IPT_ROUTE_SEPARATE);
+        b->routes.push_back(big_variant == IPT_VARIANT_GCS ? IPT_ROUTE_GCS_LAUNCH : IPT_ROUTE_SEPARATE);
     }
     // Queues (see k_batch).  Heavy work first everywhere; the members of a
     // group stay consecutive.  Groups go to the least loaded XCD queue, in
@@ -555,6 +581,15 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
     if (e == hipSuccess) e = hipMalloc(&b->d_sync, sync_bytes);
     if (e == hipSuccess) e = hipMemset(b->d_sync, 0, sync_bytes);
     if (e == hipSuccess && watchdog_word(&b->wd_host, &b->wd_addr)) e = ctl_init(b->d_sync, b->wd_addr);
+    if (e == hipSuccess && big_sync_bytes) {
+        e = hipMalloc(&b->big_sync.ptr, big_sync_bytes);
+        if (e == hipSuccess) e = hipMemset(b->big_sync.ptr, 0, big_sync_bytes);
+        if (e == hipSuccess && b->wd_addr) e = ctl_init((uint32_t*)b->big_sync.ptr, b->wd_addr);
+        b->big_sync.bytes = big_sync_bytes;
+        b->big_sync.fixed = true;
+        for (auto& x : b->big)
+            if (x.variant == IPT_VARIANT_GCS) x.tune.sync = &b->big_sync;
+    }
     if (e == hipSuccess && !insts.empty())
         e = hipMemcpy(b->d_insts, insts.data(), insts.size() * sizeof(BatchInst), hipMemcpyHostToDevice);
     if (e == hipSuccess && !items.empty())

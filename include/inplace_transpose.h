@@ -403,6 +403,8 @@ typedef struct {
 } ipt_matrix_desc;
 
 typedef struct {
+    /* n_big: instances the per-call launchers run after the persistent
+     * kernel (IPT_ROUTE_GCS_LAUNCH and IPT_ROUTE_SEPARATE) */
     int64_t count, skipped, n_instances, n_big;
     int64_t n_slice_items, n_gcs_items, n_swap_items;
     int64_t grid, lds_bytes;
@@ -429,7 +431,10 @@ typedef enum {
     IPT_ROUTE_GCS = 2,         /* cooperative groups (persistent kernel) */
     IPT_ROUTE_SWAP = 3,        /* square tile pairs (persistent kernel) */
     IPT_ROUTE_BLOCK_CYCLE = 4, /* blocks in the persistent kernel + chunk step */
-    IPT_ROUTE_SEPARATE = 5     /* per-call launchers (C2R / R2C / others) */
+    IPT_ROUTE_SEPARATE = 5,    /* per-call launchers (C2R / R2C / others) */
+    IPT_ROUTE_GCS_LAUNCH = 6   /* one separate cooperative launch (the per-call
+                                  kernel: too large for one group of the
+                                  persistent kernel, not for the per-call one) */
 } ipt_batch_route;
 /* routes: `count` int32 values (the batch's descriptor count). */
 ipt_status ipt_batch_get_routes(const ipt_batch* b, int32_t* routes);

@@ -125,12 +125,14 @@ def test_batch_block_cycle_instances(ipt, oracle):
 
 
 @pytest.mark.gpu
-def test_batch_heuristic_folds_oversized_instances(ipt, oracle):
-    """With the selector off, an instance too large for one cooperative group
-    of the persistent kernel (f64 3751 x 3723, 112 MB; f64 3970 x 2975)
-    takes the block-cycle route with its blocks in the batch, not a separate
-    C2R / R2C launch, beside ordinary instances; every instance exact after an
-    eager run and a graph replay."""
+def test_batch_heuristic_routes_oversized_instances(ipt, oracle):
+    """With the selector off, instances too large for one cooperative group
+    of the persistent kernel: f64 3970 x 2975 (94 MB) fits one group of the
+    per-call kernel and runs as one separate cooperative launch; f64 3751 x
+    3723 (112 MB) fits neither and takes the block-cycle route with its
+    blocks in the batch; neither falls to a separate C2R / R2C launch.
+    Beside ordinary instances, every instance exact after an eager run and a
+    graph replay."""
     shapes = [(3751, 3723, 1, 8), (3970, 2975, 1, 8), (512, 384, 2, 4), (33, 17, 4, 8)]
     buf, items, host = make_instances(oracle, shapes, 1300)
     ipt.selector_load(None)
@@ -138,10 +140,10 @@ def test_batch_heuristic_folds_oversized_instances(ipt, oracle):
         b = ipt.Batch(items)
     finally:
         ipt.load_default_selector()
-    routes = b.routes()
+    routes = list(b.routes())
     print("[batch heuristic oversized]", b.info(), routes, flush=True)
-    assert list(routes[:2]) == [ipt.ROUTE_BLOCK_CYCLE] * 2
-    assert b.info()["n_big"] == 0
+    assert routes[:2] == [ipt.ROUTE_BLOCK_CYCLE, ipt.ROUTE_GCS_LAUNCH]
+    assert ipt.ROUTE_SEPARATE not in routes
     b.run(graph=False)
     torch.cuda.synchronize()
     check(oracle, items, host, 1)
@@ -231,7 +233,8 @@ def test_uneven_workload_full_4096(ipt, oracle):
     b = ipt.Batch(items)
     info = b.info()
     print("[batch 4096]", info, flush=True)
-    assert info["count"] == 4096 and info["n_big"] == 0
+    assert info["count"] == 4096
+    assert ipt.ROUTE_SEPARATE not in set(b.routes())  # no C2R / R2C fallbacks
 
     def check(times):
         got_all = buf.cpu().numpy()
