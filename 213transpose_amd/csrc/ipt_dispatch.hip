@@ -34,50 +34,68 @@ k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
 
 // Out-of-place transpose (the comparator; the reference times cuTT's
 // out-of-place plan for it, src/test_cutt.cu:18-41): dst slice s (C rows x R
-// columns) = transpose of src slice s (R rows x C columns).  Tiles of TS x TS
+// columns) = transpose of src slice s (R rows x C columns).  Tiles of TR x TS
 // (TS * esz = 256 bytes) staged in LDS with an odd row pitch; VEC moves
 // 16-byte vectors (R and C multiples of 16 / esz, 16-byte aligned pointers),
-// else one element per lane.  Both sides are coalesced.
-template <typename T, bool VEC>
+// else one element per lane.  Both sides are coalesced.  Measured on MI355X
+// against the earlier form whose loads each sat behind a bounds branch (and
+// so waited for the one before): f32 7 x 4194304 977 -> 1974 GB/s, f32 4099
+// x 4093 3937 -> 4535 GB/s, the rest unchanged.
+template <typename T, bool VEC, uint32_t TR>
 __global__ void __launch_bounds__(256)
 k_oop_transpose(T* __restrict__ dst, const T* __restrict__ src, uint32_t R, uint32_t C,
                 uint64_t nslices, uint32_t nti, uint32_t ntj) {
-    constexpr uint32_t TS = 256 / sizeof(T), TP = TS + 1;
+    constexpr uint32_t TS = 256 / sizeof(T), TP = TS + 1;  // tile: TR rows x TS columns
     constexpr uint32_t PER = VEC ? 16 / sizeof(T) : 1;
-    constexpr uint32_t VPR = TS / PER, RPP = 256 / VPR;  // vectors per tile row, rows per pass
+    constexpr uint32_t VPR = TS / PER, RPP = 256 / VPR;   // loads: vectors per row, rows per pass
+    constexpr uint32_t NP = TR / RPP;
+    constexpr uint32_t VPO = TR / PER, RPO = 256 / VPO;   // stores: per output row, rows per pass
+    constexpr uint32_t NO = TS / RPO;
+    static_assert(NP >= 1 && NO >= 1 && TR % RPP == 0 && TS % RPO == 0, "tile shape");
     using V = union { uint4 v; T e[16 / sizeof(T)]; };
-    __shared__ T tile[TS * TP];
+    __shared__ T tile[TR * TP];
     const uint64_t per_slice = (uint64_t)nti * ntj, total = per_slice * nslices;
     const uint32_t x = (threadIdx.x % VPR) * PER, y0 = threadIdx.x / VPR;
+    const uint32_t xo = (threadIdx.x % VPO) * PER, yo0 = threadIdx.x / VPO;
     for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const uint64_t sl = w / per_slice, t = w - sl * per_slice;
-        const uint32_t r0 = (uint32_t)(t / ntj) * TS, c0 = (uint32_t)(t % ntj) * TS;
+        const uint32_t r0 = (uint32_t)(t / ntj) * TR, c0 = (uint32_t)(t % ntj) * TS;
         const T* sp = src + sl * (uint64_t)R * C;
         T* dp = dst + sl * (uint64_t)R * C;
-        for (uint32_t y = y0; y < TS; y += RPP) {
-            const uint32_t r = r0 + y, c = c0 + x;
-            if (r < R && c < C) {
-                if constexpr (VEC) {
-                    V u;
-                    u.v = *reinterpret_cast<const uint4*>(sp + (uint64_t)r * C + c);
+        // Every pass's load first, at addresses clamped into the slice (a
+        // branch around each load would wait for the previous one): the
+        // tile's loads are all in flight at once; lanes outside the matrix
+        // then drop what they loaded.
+        const uint32_t cc = min(c0 + x, C - PER);
+        V u[NP];
 #pragma unroll
-                    for (uint32_t q = 0; q < PER; q++) tile[y * TP + x + q] = u.e[q];
-                } else {
-                    tile[y * TP + x] = sp[(uint64_t)r * C + c];
-                }
+        for (uint32_t p = 0; p < NP; p++) {
+            const uint32_t r = min(r0 + y0 + p * RPP, R - 1);
+            if constexpr (VEC) u[p].v = *reinterpret_cast<const uint4*>(sp + (uint64_t)r * C + cc);
+            else u[p].e[0] = sp[(uint64_t)r * C + cc];
+        }
+        const bool col_in = c0 + x < C;
+#pragma unroll
+        for (uint32_t p = 0; p < NP; p++) {
+            const uint32_t y = y0 + p * RPP;
+            if (col_in && r0 + y < R) {
+#pragma unroll
+                for (uint32_t q = 0; q < PER; q++) tile[y * TP + x + q] = u[p].e[q];
             }
         }
         __syncthreads();
-        for (uint32_t y = y0; y < TS; y += RPP) {  // output row c0 + y, columns r0 + x ..
-            const uint32_t orow = c0 + y, ocol = r0 + x;
+#pragma unroll
+        for (uint32_t p = 0; p < NO; p++) {  // output row c0 + y, columns r0 + xo ..
+            const uint32_t y = yo0 + p * RPO;
+            const uint32_t orow = c0 + y, ocol = r0 + xo;
             if (orow < C && ocol < R) {
                 if constexpr (VEC) {
-                    V u;
+                    V o;
 #pragma unroll
-                    for (uint32_t q = 0; q < PER; q++) u.e[q] = tile[(x + q) * TP + y];
-                    *reinterpret_cast<uint4*>(dp + (uint64_t)orow * R + ocol) = u.v;
+                    for (uint32_t q = 0; q < PER; q++) o.e[q] = tile[(xo + q) * TP + y];
+                    *reinterpret_cast<uint4*>(dp + (uint64_t)orow * R + ocol) = o.v;
                 } else {
-                    dp[(uint64_t)orow * R + ocol] = tile[x * TP + y];
+                    dp[(uint64_t)orow * R + ocol] = tile[xo * TP + y];
                 }
             }
         }
@@ -708,22 +726,29 @@ ipt_status run_pass(DeviceCtx* ctx, void* data, int esz, int64_t R, int64_t C, i
     return run_pass_ix<uint64_t, uint64_t>(ctx, (uint64_t*)data, h, d3, map_id, t, stream);
 }
 
-template <typename T>
-static ipt_status launch_oop(DeviceCtx* ctx, T* dst, const T* src, const Shape& s, hipStream_t st) {
+template <typename T, uint32_t TR>
+static ipt_status launch_oop_tr(DeviceCtx* ctx, T* dst, const T* src, const Shape& s, hipStream_t st) {
     constexpr uint32_t TS = 256 / sizeof(T), PER = 16 / sizeof(T);
     const uint32_t R = (uint32_t)s.d2, C = (uint32_t)s.d1;
-    const uint32_t nti = (R + TS - 1) / TS, ntj = (C + TS - 1) / TS;
+    const uint32_t nti = (R + TR - 1) / TR, ntj = (C + TS - 1) / TS;
     const uint64_t tiles = (uint64_t)nti * ntj * (uint64_t)s.d3;
     const int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ncu_of(ctx) * 8);
     const bool vec = R % PER == 0 && C % PER == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0;
     if (vec)
-        hipLaunchKernelGGL((k_oop_transpose<T, true>), dim3(grid), dim3(256), 0, st, dst, src, R, C,
-                           (uint64_t)s.d3, nti, ntj);
+        hipLaunchKernelGGL((k_oop_transpose<T, true, TR>), dim3(grid), dim3(256), 0, st, dst, src, R,
+                           C, (uint64_t)s.d3, nti, ntj);
     else
-        hipLaunchKernelGGL((k_oop_transpose<T, false>), dim3(grid), dim3(256), 0, st, dst, src, R, C,
-                           (uint64_t)s.d3, nti, ntj);
+        hipLaunchKernelGGL((k_oop_transpose<T, false, TR>), dim3(grid), dim3(256), 0, st, dst, src,
+                           R, C, (uint64_t)s.d3, nti, ntj);
     IPT_LAUNCH_CHECK("k_oop_transpose");
     return IPT_OK;
+}
+
+template <typename T>
+static ipt_status launch_oop(DeviceCtx* ctx, T* dst, const T* src, const Shape& s, hipStream_t st) {
+    // square tiles: half-height ones gained under 10 % on 12 MB slices (whose
+    // calls are launch-bound) and lost up to 25 % on skinny and large ones
+    return launch_oop_tr<T, 256 / sizeof(T)>(ctx, dst, src, s, st);
 }
 
 ipt_status run_oop(DeviceCtx* ctx, void* dst, const void* src, const Shape& s, hipStream_t st) {
