@@ -385,8 +385,10 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
         } else {
             const uint64_t TS = s.esz == 4 ? 64 : 32;
             const bool vec = ((uint64_t)s.d1 * s.esz) % 16 == 0 && (uintptr_t)data % 16 == 0;
-            const bool rs = (uint64_t)s.d1 * s.d1 * s.esz < ((uint64_t)1 << 31);
-            I.kind = vec ? (rs ? BK_SWAP_RS : BK_SWAP_VEC) : BK_SWAP;
+            // the buffer-resource form takes any side for 4-byte elements (as
+            // the per-call square swap)
+            const bool rs = (vec || s.esz == 4) && (uint64_t)s.d1 * s.d1 * s.esz < ((uint64_t)1 << 31);
+            I.kind = rs ? BK_SWAP_RS : vec ? BK_SWAP_VEC : BK_SWAP;
             const uint64_t nt = ((uint64_t)s.d1 + TS - 1) / TS;
             const uint64_t pairs = nt * (nt + 1) / 2;
             if (pairs >= ((uint64_t)1 << 32)) return IPT_ERR_TOO_LARGE;

@@ -365,8 +365,11 @@ static ipt_status run_square_ts(DeviceCtx* ctx, T* data, const Shape& s, const T
     const uint64_t pairs = (uint64_t)nt * (nt + 1) / 2;
     const bool vec = ((uint64_t)N * sizeof(T)) % 16 == 0 && (uintptr_t)data % 16 == 0;
     // Slices below 2^31 bytes go through a buffer resource (32-bit offsets),
-    // one grid round of resident workgroups (measured best on 16384^2 f32).
-    const bool rs = vec && (uint64_t)N * N * sizeof(T) < ((uint64_t)1 << 31);
+    // one grid round of resident workgroups (measured best on 16384^2 f32):
+    // 16-byte rows, and 4-byte elements with any side (f32 4095^2 4565 ->
+    // 4754 GB/s, 1001^2 x 8 3466 -> 4041 over the scalar form).  Odd f64
+    // sides keep the scalar form (2049^2 4690 vs 4389 through the resource).
+    const bool rs = (vec || sizeof(T) == 4) && (uint64_t)N * N * sizeof(T) < ((uint64_t)1 << 31);
     auto k = rs ? k_square_swap_rs<T, TS> : vec ? k_square_swap<T, TS, true> : k_square_swap<T, TS, false>;
     int grid;
     grid_for(ctx, k, kSwapBlock, 0, pairs * (uint64_t)s.d3, rs ? 1 : t.grid_mult, &grid);

@@ -753,11 +753,17 @@ struct WaveTile {
     }
 };
 
-// Square swap through a buffer resource (slices below 2^31 bytes, both the
-// side and the base 16-byte aligned): the same tile-pair exchange as
-// swap_body with branch-free 16-byte loads (lanes outside the slice load
-// nothing) and device-scope stores.  Measured on a 16384^2 f32 matrix (HBM):
-// 4705 -> 4819 GB/s.
+// Square swap through a buffer resource (slices below 2^31 bytes, any side
+// and element-aligned base): the same tile-pair exchange as swap_body with
+// branch-free 16-byte loads (lanes outside the slice load nothing) and
+// device-scope stores.  Measured on a 16384^2 f32 matrix (HBM): 4705 -> 4819
+// GB/s.  Sides that are not a multiple of 16 bytes work as in WaveTile:
+// buffer accesses need only element alignment on gfx950 and are range
+// checked per dword, so a micro-row that runs past its matrix row loads
+// harmless values (they land in positions past the row, never stored), and
+// in the last tile column the stores of such a micro-row go element by
+// element up to the row end.  (Only the last tile column has them: I <= J,
+// so a pair whose row band is the last one is a diagonal pair.)
 template <typename T, int TS, int BLOCK>
 __device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, uint64_t t, T* lds) {
     constexpr uint32_t TP = TS + 1, PER = 16 / sizeof(T), VPR = TS / PER, VPT = TS * VPR;
@@ -810,10 +816,28 @@ __device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, u
             }
             const bool oka = r0 + y < N && c0 + x < N;
             const bool okb = !diag && c0 + y < N && r0 + x < N;
-            __builtin_amdgcn_raw_buffer_store_b128(o.v, rs, oka ? ((r0 + y) * N + c0 + x) * E : kOutside,
-                                                   0, kStoreDevice);
-            __builtin_amdgcn_raw_buffer_store_b128(p.v, rs, okb ? ((c0 + y) * N + r0 + x) * E : kOutside,
-                                                   0, kStoreDevice);
+            const bool fa = c0 + x + PER <= N, fb = r0 + x + PER <= N;  // whole micro-row inside
+            const uint32_t offa = ((r0 + y) * N + c0 + x) * E, offb = ((c0 + y) * N + r0 + x) * E;
+            __builtin_amdgcn_raw_buffer_store_b128(o.v, rs, oka && fa ? offa : kOutside, 0, kStoreDevice);
+            __builtin_amdgcn_raw_buffer_store_b128(p.v, rs, okb && fb ? offb : kOutside, 0, kStoreDevice);
+            if (c0 + TS > N) {  // the last tile column: partial micro-rows (uniform per pair)
+#pragma unroll
+                for (uint32_t q = 0; q < PER; q++) {
+                    const bool wa = oka && !fa && c0 + x + q < N;
+                    const bool wb = okb && !fb && r0 + x + q < N;
+                    if constexpr (sizeof(T) == 4) {
+                        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)o.e[q], rs, wa ? offa + q * E : kOutside,
+                                                              0, kStoreDevice);
+                        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)p.e[q], rs, wb ? offb + q * E : kOutside,
+                                                              0, kStoreDevice);
+                    } else {
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (uint64_t)o.e[q]), rs,
+                                                              wa ? offa + q * E : kOutside, 0, kStoreDevice);
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (uint64_t)p.e[q]), rs,
+                                                              wb ? offb + q * E : kOutside, 0, kStoreDevice);
+                    }
+                }
+            }
         }
     }
 }
@@ -1093,7 +1117,7 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
                                                               I.ntiles, it.sub, bar, ctl, l, I.ord);
         break;
     }
-    case BK_SWAP_RS:  // buffer-resource form: slices below 2^31 bytes, 16-byte rows
+    case BK_SWAP_RS:  // buffer-resource form: slices below 2^31 bytes, any side
         swap_body_rs<T, TS_SWAP, kBatchBlock>(base, I.R, it.sub, lds);
         break;
     case BK_SWAP_VEC:

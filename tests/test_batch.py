@@ -43,6 +43,7 @@ SHAPES = [  # (d1, d2, d3, esz): slice-in-LDS, GCS, square-swap and big routes, 
     (512, 384, 2, 4), (256, 1000, 1, 8), (1000, 999, 1, 4), (3000, 1000, 1, 4), (97, 203, 5, 8),
     (1024, 1024, 1, 4), (2048, 2048, 1, 8), (4096, 40, 1, 4), (40, 4096, 1, 8), (129, 131, 1, 4),
     (2, 3000, 2, 8), (3000, 2, 1, 4), (700, 700, 2, 4), (255, 257, 1, 8), (4096, 4096, 1, 4),
+    (129, 129, 2, 4), (333, 333, 1, 8), (1001, 1001, 1, 4),
 ]
 
 

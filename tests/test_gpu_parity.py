@@ -70,6 +70,32 @@ def test_each_variant_matches_oracle(ipt, oracle, variant, esz):
     assert ran > 0
 
 
+@pytest.mark.parametrize("esz", [4, 8])
+def test_square_swap_any_side_and_base(ipt, oracle, esz):
+    """Square swap on sides that are not a multiple of 16 bytes (partial
+    micro-rows in the last tile column) and bases off 16-byte alignment by
+    1-3 elements, several slices: the neighbours of the slice are untouched."""
+    L = ipt.lib()
+    import ctypes
+    for k, (n, d3, off) in enumerate([(63, 3, 1), (65, 2, 0), (97, 1, 3), (129, 2, 2), (255, 1, 1),
+                                      (1001, 1, 0), (1001, 2, 3), (2049, 1, 1)]):
+        if esz == 8 and n == 2049:
+            d3 = 2
+        print(f"[case] square n={n} d3={d3} esz={esz} off={off}", flush=True)
+        m = n * n * d3
+        a = oracle.random_bits(m + 8, esz, 700 + k)
+        t = to_dev(a)
+        view = t[off:off + m]
+        st = L.ipt_transpose_variant(ctypes.c_void_p(view.data_ptr()), esz, n, n, d3,
+                                     ipt.VARIANT_SQUARE_SWAP, None,
+                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        assert st == 0, (st, L.ipt_last_error())
+        torch.cuda.synchronize()
+        exp = a.copy()
+        exp[off:off + m] = oracle.seq_tt_213(a[off:off + m], n, n, d3)
+        assert np.array_equal(to_host(t, esz), exp), (n, d3, esz, off)
+
+
 BLOCK_CYCLE_SHAPES = [  # (d1, d2, d3): skinny wide / tall, then large rectangles
     (65536, 32, 1), (32, 65536, 1), (7, 1 << 18, 2), (1 << 16, 5, 3), (128, 4096, 2),
     (4096, 128, 2), (3, 1 << 19, 1), (1 << 19, 3, 1), (2000, 64, 2), (64, 2000, 2),
