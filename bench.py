@@ -97,6 +97,10 @@ def parse():
                     help="one reference config shape (1, 2a, ..., 3d): calibration copies of "
                          "its bytes, then warmup + steps transposes with the default dispatch "
                          "(for profilers)")
+    ap.add_argument("--profile-uniform", default=None, metavar="D1xD2xESZ",
+                    help="a batch of copies of one row-major shape (d1 columns, d2 rows, esz "
+                         "bytes; about 4 GB in all), run eagerly warmup + steps times (for "
+                         "profilers: row alignment against traffic counters)")
     ap.add_argument("--batch-routes", action="store_true",
                     help="the uneven batch's time by route and by slice size: the workload's "
                          "instances of each route run as a batch of their own")
@@ -321,6 +325,26 @@ def uneven_items(torch, ipt, count=BATCH_COUNT, seed=BATCH_SEED):
                       d1, d2, d3))
     payload = sum(d1 * d2 * d3 * esz for d1, d2, d3, esz in shapes)
     return buf, items, payload
+
+
+def profile_uniform_main(args, torch, ipt):
+    """A batch of identical instances (one shape, 256-byte aligned, about
+    4 GB in all), run eagerly warmup + steps times: each run is one k_batch
+    dispatch over the same work (runs under rocprofv3)."""
+    d1, d2, esz = (int(x) for x in args.profile_uniform.lower().split("x"))
+    n = max(1, int(4e9 // (d1 * d2 * esz)))
+    pitch = (d1 * d2 * esz + 255) // 256 * 256
+    buf = torch.zeros(n * pitch // 8 + 1, dtype=torch.int64, device="cuda")
+    raw = buf.view(torch.uint8)
+    dt = torch.int32 if esz == 4 else torch.int64
+    items = [(raw[i * pitch:i * pitch + d1 * d2 * esz].view(dt), d1, d2, 1) for i in range(n)]
+    b = ipt.Batch(items)
+    for _ in range(args.warmup + args.steps):
+        b.run(graph=False)
+    torch.cuda.synchronize()
+    b.check()
+    b.close()
+    print(f"profile run done: {n} x {d1}x{d2}x{esz}", flush=True)
 
 
 def profile_batch_main(args, torch, ipt):
@@ -649,6 +673,9 @@ def main():
         return
     if args.profile_config:
         profile_config_main(args, torch, ipt)
+        return
+    if args.profile_uniform:
+        profile_uniform_main(args, torch, ipt)
         return
     if args.batch_routes:
         batch_routes_main(args, torch, ipt)
