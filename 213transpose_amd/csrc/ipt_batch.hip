@@ -140,7 +140,7 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
                    const Tune& t, int* big_variant) {
     if (s.d1 == 1 || s.d2 == 1) return Route::Skip;
     const int64_t slice_bytes = s.d2 * slice_pitch(s.d1) * s.esz;
-    const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, kBatchJrClass);
+    const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, kBatchJrClass, false, true);
     const bool gcs_ok = gg.ok && (int64_t)gg.g <= grid && gg.lds <= (size_t)lds_cap;
     switch (forced) {
     case IPT_VARIANT_SLICE_LDS: return slice_bytes <= lds_cap ? Route::Slice : Route::Big;
@@ -182,7 +182,7 @@ int batch_choice(const DeviceCtx* ctx, const Shape& s, const float* y, int64_t l
     if (tuned >= 0) return tuned;
     if (!y || !selector_domain(s.d1, s.d2, s.d3, s.esz) || s.d1 == s.d2) return -1;
     const int64_t slice_bytes = s.d2 * slice_pitch(s.d1) * s.esz;
-    const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, kBatchJrClass);
+    const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, kBatchJrClass, false, true);
     const bool route_ok[4] = {slice_bytes <= lds_cap, s.d1 == s.d2,
                               gg.ok && (int64_t)gg.g <= grid, gg.ok && (int64_t)gg.g <= grid};
     int best = -1;
@@ -364,9 +364,9 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             }
             b->info.n_slice_items += ((uint64_t)s.d3 + per - 1) / per;
         } else if (p == Route::Gcs) {
-            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, kBatchJrClass);
+            const GcsGeom gg = gcs_geom(ctx, s, lds_cap, false, kBatchJrClass, false, true);
             I.kind = BK_GCS;
-            I.jr = (uint32_t)gcs_jr_class(s.esz, gg.K, gg.KL, kBatchBlock / 64);
+            I.jr = (uint32_t)gcs_jr_class(s.esz, gg.K, gg.KL, kBatchBlock / 64, rows_off_lines(s));
             I.g = gg.g;
             I.K = gg.K;
             I.KL = gg.KL;

@@ -160,7 +160,7 @@ static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
 //
 // gcs_geom_cls: the geometry with register tiles of class jr_cls.
 static GcsGeom gcs_geom_cls(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread,
-                            int jr_cls, bool pack) {
+                            int jr_cls, bool pack, bool tail) {
     GcsGeom gg{};
     const int64_t R = s.d2, C = s.d1, esz = s.esz;
     gg.ts = esz == 4 ? 32 : 16;
@@ -170,7 +170,7 @@ static GcsGeom gcs_geom_cls(const DeviceCtx* ctx, const Shape& s, int64_t lds_ca
     const int64_t ntiles = nti * ntj;
     constexpr int64_t nw = kGcsBlock / 64;
     const int64_t klmax = std::min<int64_t>(cap / tile_bytes, gcs_wave_lds_tiles(esz) * nw);
-    const int64_t krmax = (int64_t)gcs_jr_of(esz, jr_cls) * nw;
+    const int64_t krmax = (int64_t)(gcs_jr_of(esz, jr_cls) - (tail ? 1 : 0)) * nw;
     gg.ok = klmax >= 1 && R * C * esz < ((int64_t)1 << 31);
     if (!gg.ok) return gg;
     const int64_t ncu = ncu_of(ctx);
@@ -198,12 +198,13 @@ static GcsGeom gcs_geom_cls(const DeviceCtx* ctx, const Shape& s, int64_t lds_ca
 }
 
 GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread, int jr_cls,
-                 bool pack) {
-    if (jr_cls <= 1) return gcs_geom_cls(ctx, s, lds_cap, spread, jr_cls, pack);
-    const GcsGeom small = gcs_geom_cls(ctx, s, lds_cap, spread, 1, pack);
-    if (!small.ok) return gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack);
+                 bool pack, bool tail) {
+    tail = tail && rows_off_lines(s);
+    if (jr_cls <= 1) return gcs_geom_cls(ctx, s, lds_cap, spread, jr_cls, pack, tail);
+    const GcsGeom small = gcs_geom_cls(ctx, s, lds_cap, spread, 1, pack, tail);
+    if (!small.ok) return gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack, false);
     if (!pack || small.grid == 0 || small.grid / small.g != 1) return small;
-    const GcsGeom big = gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack);
+    const GcsGeom big = gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack, false);
     return big.ok && big.grid && big.grid / big.g >= 2 ? big : small;
 }
 

@@ -171,8 +171,14 @@ struct GcsGeom {
 // (gcs_jr_of; the batch kernel stops at kBatchJrClass).
 // pack: when slices take several rounds, size groups so that a whole number
 // of them fills the CUs, and launch exactly those workgroups.
+// tail: rows off the 128-byte line grid keep one register tile per wave free
+// (the line after a wave's run of tiles in the line-aligned loads of
+// gcs_body); classes below the largest only.
 GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true,
-                 int jr_cls = 2, bool pack = false);
+                 int jr_cls = 2, bool pack = false, bool tail = false);
+// Whether rows of this shape are off the 128-byte line grid (the line-aligned
+// loads then want the tail register tile).
+inline bool rows_off_lines(const Shape& s) { return ((s.d1 * s.esz) & 127) != 0; }
 int64_t slice_pitch(int64_t C);
 int64_t lds_max_of(const DeviceCtx* ctx);
 int ncu_of(const DeviceCtx* ctx);

@@ -530,7 +530,7 @@ namespace ipt {
 // Sync state (device memory, zero at rest; each launch restores it):
 //   ctl[2] watchdog flag (sticky); ctl[6..7] the host watchdog word's address
 //   cnt[bar_words(g) * S ...] the barrier lines of slice S (see
-//   group_arrive_wait); no two slices share a line.
+//   group_arrive / group_wait); no two slices share a line.
 // The spin is bounded: after ~2^26 polls the member records ctl[2] and
 // proceeds, so a broken invariant shows up as an error, never as a hang.
 // ctl[3] (a poll limit in place of 2^26) and ctl[4] (arrivals to wait for
@@ -581,17 +581,22 @@ constexpr uint32_t kBarLines = 17;
 // Words of barrier state per slice for a group of g.
 IPT_HD uint64_t bar_words(uint32_t g) { return 32ull * (1 + (g < kBarLines - 1 ? g : kBarLines - 1)); }
 
-__device__ __forceinline__ bool group_arrive_wait(uint32_t* bar, uint32_t m, uint32_t g,
-                                                  uint32_t extra, uint32_t limit) {
+// The two halves of the barrier (thread 0 of a member): arrive, then wait
+// until every member has arrived (false on watchdog).  A member may do work
+// of its own in between, as long as it needs nothing from the others.
+__device__ __forceinline__ void group_arrive(uint32_t* bar, uint32_t m, uint32_t g) {
     if (g <= kBarLines - 1) {
         atomic_add_agent(&bar[0], 1u);
-        return spin_until(&bar[0], g + extra, limit);
+        return;
     }
     const uint32_t nsub = kBarLines - 1;
     const uint32_t q = m % nsub;
     const uint32_t members = g / nsub + (q < g % nsub ? 1u : 0u);
     if (atomic_add_agent(&bar[32 * (1 + q)], 1u) == members - 1) atomic_add_agent(&bar[0], 1u);
-    return spin_until(&bar[0], nsub + extra, limit);
+}
+
+__device__ __forceinline__ bool group_wait(uint32_t* bar, uint32_t g, uint32_t extra, uint32_t limit) {
+    return spin_until(&bar[0], (g <= kBarLines - 1 ? g : kBarLines - 1) + extra, limit);
 }
 
 __device__ __forceinline__ void group_depart(uint32_t* bar, uint32_t g) {
@@ -604,8 +609,9 @@ __device__ __forceinline__ void group_depart(uint32_t* bar, uint32_t g) {
 }
 
 // Staging shared by the members of one slice: after every wave has its
-// loads back, thread 0 arrives and waits for the group; `bar` is null when
-// the slice has one member.
+// loads back, thread 0 arrives, and before the first store it waits for
+// the group (group_arrive / group_wait); `bar` is null when the slice has
+// one member.
 // The watchdog settings ctl[3], ctl[4], read by thread 0 before the member's
 // loads are issued (so the read overlaps them instead of delaying the
 // arrival; nothing writes them while a kernel runs).
@@ -630,17 +636,6 @@ __device__ __forceinline__ void report_watchdog(uint32_t* ctl) {
     const uint64_t a = ((uint64_t)ctl[7] << 32) | ctl[6];
     if (a) __hip_atomic_store(reinterpret_cast<uint32_t*>(a), 1u, __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
-__device__ __forceinline__ void group_sync(uint32_t* bar, uint32_t* ctl, uint32_t m, uint32_t g,
-                                           const Watchdog& wd) {
-    wait_vmem();
-    __syncthreads();
-    if (g > 1) {
-        if (threadIdx.x == 0 && !group_arrive_wait(bar, m, g, wd.extra, wd.limit))
-            report_watchdog(ctl);
-        __syncthreads();
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -705,6 +700,58 @@ struct WaveTile {
         for (uint32_t k = 0; k < PER; k++)
             v[k] = __builtin_amdgcn_raw_buffer_load_b128(
                 rs, ok ? o + k * cols * (uint32_t)sizeof(T) : kOutside, 0, 0);
+    }
+    // Line-aligned form of load(): each lane loads 16-byte chunk lane % 8 of
+    // the 128-byte line that holds the start of its row's segment, through
+    // `rl`, a resource over the slice from the line below its base (`mis` =
+    // the base's offset in that line, bytes).  realign() then moves the bytes
+    // into place, given the lines of the right-hand neighbour tile.
+    __device__ __forceinline__ void load_lines(Rsrc rl, uint32_t mis, uint32_t r0, uint32_t c0,
+                                               uint32_t rows, uint32_t cols, uint32_t lane, bool live) {
+        const uint32_t r = r0 + PER * (lane >> 3), chunk = 16 * (lane & 7);
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            const uint32_t a = mis + ((r + k) * cols + c0) * (uint32_t)sizeof(T);
+            v[k] = __builtin_amdgcn_raw_buffer_load_b128(
+                rl, live && r + k < rows ? (a & ~127u) + chunk : kOutside, 0, 0);
+        }
+    }
+    // x rotated down by t dwords (t uniform): dword q of the result is
+    // x[(q + t) mod 4].
+    static __device__ __forceinline__ u32x4 rotate_dwords(u32x4 x, uint32_t t) {
+        if (t & 1) x = u32x4{x[1], x[2], x[3], x[0]};
+        if (t & 2) x = u32x4{x[2], x[3], x[0], x[1]};
+        return x;
+    }
+    // After load_lines() of this tile and of `nx` (the tile at (r0, c0 + TS),
+    // whose lines are one line further along each row), put into v[] what
+    // load() would have loaded: row bytes [d, d + 128) of the two lines, d
+    // the segment's offset in its line (values past the row end or the
+    // slice end differ from load()'s; such positions are never stored).
+    // Lane (a, b) wants chunk b + d / 16 of the 16 (this tile's 8, then
+    // nx's), shifted by (d / 4) mod 4 dwords.  That dword shift is the same
+    // for every lane of one load (its rows are PER apart and PER * cols * E
+    // is a multiple of 16), so only the source lane varies: one ds_bpermute
+    // per dword, from lanes that offer nx's dword when their chunk index is
+    // below d / 16 plus the carry into the next chunk.
+    __device__ __forceinline__ void realign(const WaveTile& nx, uint32_t mis, uint32_t r0, uint32_t c0,
+                                            uint32_t cols, uint32_t lane) {
+        constexpr uint32_t E = sizeof(T);
+        const uint32_t a = lane >> 3, b = lane & 7;
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            const uint32_t d = (mis + ((r0 + PER * a + k) * cols + c0) * E) & 127u;
+            const uint32_t s = d >> 4;
+            const uint32_t t = __builtin_amdgcn_readfirstlane((d >> 2) & 3u);
+            const u32x4 cur = rotate_dwords(v[k], t), nxt = rotate_dwords(nx.v[k], t);
+#pragma unroll
+            for (uint32_t q = 0; q < 4; q++) {
+                const uint32_t e = q + t >= 4 ? 1u : 0u;  // carried into the next chunk
+                const uint32_t src = (lane & ~7u) | ((b + s + e) & 7u);
+                const uint32_t y = b < s + e ? nxt[q] : cur[q];
+                v[k][q] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)y);
+            }
+        }
     }
     __device__ __forceinline__ void transpose() {
         union U { u32x4 v; T e[PER]; };
@@ -898,10 +945,11 @@ constexpr int kBatchJrClass = 1;
 constexpr int gcs_jr_of(int esz, int cls) {
     return cls == 0 ? 1 : cls == 1 ? gcs_wave_reg_tiles(esz) / 2 : gcs_wave_reg_tiles(esz);
 }
-inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw) {
-    const uint32_t extra = K - KL;
-    if (extra <= (uint32_t)gcs_jr_of(esz, 0) * nw) return 0;
-    if (extra <= (uint32_t)gcs_jr_of(esz, 1) * nw) return 1;
+// tail: classes 0 and 1 keep one register tile per wave free (gcs_geom).
+inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw, bool tail = false) {
+    const uint32_t extra = K - KL, t = tail ? 1u : 0u;
+    if (extra <= ((uint32_t)gcs_jr_of(esz, 0) - t) * nw) return 0;
+    if (extra <= ((uint32_t)gcs_jr_of(esz, 1) - t) * nw) return 1;
     return 2;
 }
 
@@ -948,7 +996,19 @@ inline TileOrder banded_order(uint32_t nw, uint32_t nti, uint32_t ntj) {
 // (already transposed, lane-linear: conflict-free 16-byte accesses, and
 // each wave reads back only what it wrote); tiles kl <= k < nk stay in
 // registers across the group barrier.
-template <typename T, int BLOCK, int JR, int SP>
+//
+// Rows whose pitch (or base) is off the 128-byte line grid split every row
+// segment of a tile over two lines; loading them that way costs 14-16 % of
+// the batch kernel's time on such shapes.  With AL (the batch kernel), a
+// wave whose tiles are one run of neighbours along a tile row (banded
+// order, inside one full band) loads whole lines instead
+// (WaveTile::load_lines, plus the line after its last tile into a register
+// tile the planner leaves free: gcs_geom's `tail`) and moves the bytes into
+// place in registers (WaveTile::realign) after its member has arrived at
+// the group barrier, so the shuffling overlaps the wait for the other
+// members.  (Single launches keep plain loads: every member loads at once
+// there and the wait is short.)
+template <typename T, int BLOCK, int JR, int SP, bool AL = true>
 __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
                                           uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
                                           uint32_t m, uint32_t* bar, uint32_t* ctl, u32x4* lds,
@@ -981,24 +1041,81 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
         *r0 = ti * W::TS;
         *c0 = tj * W::TS;
     };
+    // Line-aligned loads (AL; `sw`, the same for the whole member): rows off
+    // the 128-byte line grid in the banded order.  Then wave w's tiles are
+    // k = w + NW * i, the first jl of them (k < kl) staged in LDS and the
+    // rest in register tiles j = i - jl, so each wave's tiles are one
+    // sequence (a wave still holds at most ceil((nk - kl) / NW) <= JR
+    // register tiles); otherwise register tile j of wave w is k = kl + w +
+    // NW * j.  A wave goes line-aligned (al) when its live tiles i < n lie in
+    // one full band, so they are the neighbours (ar0, ac0 + i * TS), and a
+    // register tile is left after them (n - jl < JR): it takes the line
+    // after the run, the tail.
+    constexpr uint32_t E = sizeof(T);
+    const uint32_t mis = (uint32_t)reinterpret_cast<uintptr_t>(base) & 127u;
+    const bool sw = AL && ord.bh == NW && (mis != 0 || ((C * E) & 127u) != 0) &&
+                    (uint64_t)R * C * E + 256 < ((uint64_t)1 << 31);
+    const uint32_t jl = kl > wave ? (kl - wave + NW - 1) / NW : 0u;
+    auto rk = [&](int j) { return sw ? wave + NW * (jl + (uint32_t)j) : kl + wave + NW * (uint32_t)j; };
+    uint32_t n = 0, ar0 = 0, ac0 = 0;
+    bool al = false;
+    if constexpr (AL) {
+        n = wave < nk ? (nk - wave + NW - 1) / NW : 0u;
+        if (sw && n > 0 && n < jl + JR) {
+            const uint32_t p0 = m * K + wave;
+            const uint32_t b0 = ord.dband.div(p0), b1 = ord.dband.div(p0 + NW * (n - 1));
+            al = b0 == b1 && b0 < ord.full;
+            if (al) origin(wave, &ar0, &ac0);
+        }
+    }
     W rt[JR], lt[JL];
+    if (al) {
+        const Rsrc rl = make_rsrc(reinterpret_cast<const char*>(base) - mis, mis + R * C * E);
 #pragma unroll
-    for (int j = 0; j < JR; j++) {
-        const uint32_t k = kl + wave + j * NW;
-        uint32_t r0, c0;
-        origin(k, &r0, &c0);
-        rt[j].load(rs, r0, c0, R, C, lane, k < nk);
+        for (int j = 0; j < JR; j++)
+            rt[j].load_lines(rl, mis, ar0, ac0 + (jl + j) * W::TS, R, C, lane, jl + j <= n);
+#pragma unroll
+        for (int j = 0; j < JL; j++)
+            lt[j].load_lines(rl, mis, ar0, ac0 + j * W::TS, R, C, lane, (uint32_t)j < jl);
+    } else {
+#pragma unroll
+        for (int j = 0; j < JR; j++) {
+            const uint32_t k = rk(j);
+            uint32_t r0, c0;
+            origin(k, &r0, &c0);
+            rt[j].load(rs, r0, c0, R, C, lane, k < nk);
+        }
+#pragma unroll
+        for (int j = 0; j < JL; j++) {
+            const uint32_t k = wave + j * NW;
+            uint32_t r0, c0;
+            origin(k, &r0, &c0);
+            lt[j].load(rs, r0, c0, R, C, lane, k < kl);
+        }
+    }
+    // Without line-aligned loads in the member, waves transpose their tiles
+    // and stage their LDS tiles as the loads come back (overlapping the later
+    // loads), before the member arrives at the barrier.  With them, the
+    // member arrives first and then its line-aligned waves realign each tile
+    // with tile i + 1's lines (still raw: tiles go in order; after the last
+    // live one, the tail), so the shuffling overlaps the wait for the other
+    // members; an LDS tile goes to LDS as soon as it is in place, which frees
+    // its registers.
+    if (sw) {
+        wait_vmem();
+        __syncthreads();
+        if (g > 1 && threadIdx.x == 0) group_arrive(bar, m, g);
     }
 #pragma unroll
     for (int j = 0; j < JL; j++) {
         const uint32_t k = wave + j * NW;
-        uint32_t r0, c0;
-        origin(k, &r0, &c0);
-        lt[j].load(rs, r0, c0, R, C, lane, k < kl);
-    }
+        if (al && (uint32_t)j < jl) {
+            const W& lnext = lt[j + 1 < JL ? j + 1 : j];
+            W nx;
 #pragma unroll
-    for (int j = 0; j < JL; j++) {
-        const uint32_t k = wave + j * NW;
+            for (uint32_t q = 0; q < W::PER; q++) nx.v[q] = (uint32_t)j + 1 < jl ? lnext.v[q] : rt[0].v[q];
+            lt[j].realign(nx, mis, ar0, ac0 + j * W::TS, C, lane);
+        }
         if (k < kl) {
             lt[j].transpose();
 #pragma unroll
@@ -1006,8 +1123,18 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
         }
     }
 #pragma unroll
-    for (int j = 0; j < JR; j++) rt[j].transpose();
-    group_sync(bar, ctl, m, g, wd);
+    for (int j = 0; j < JR; j++) {
+        if (j + 1 < JR && al && jl + j < n)
+            rt[j].realign(rt[j + 1 < JR ? j + 1 : j], mis, ar0, ac0 + (jl + j) * W::TS, C, lane);
+        rt[j].transpose();
+    }
+    if (!sw) {
+        wait_vmem();
+        __syncthreads();
+        if (g > 1 && threadIdx.x == 0) group_arrive(bar, m, g);
+    }
+    if (g > 1 && threadIdx.x == 0 && !group_wait(bar, g, wd.extra, wd.limit)) report_watchdog(ctl);
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < JL; j++) {
         const uint32_t k = wave + j * NW;
@@ -1022,7 +1149,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     }
 #pragma unroll
     for (int j = 0; j < JR; j++) {
-        const uint32_t k = kl + wave + j * NW;
+        const uint32_t k = rk(j);
         uint32_t r0, c0;
         origin(k, &r0, &c0);
         rt[j].template store_t<SP>(rs, r0, c0, R, C, lane, k < nk);
@@ -1044,8 +1171,11 @@ k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g
     for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd, g); slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
-        gcs_body<T, BLOCK, JR, SP>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
-                                   g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds, ord);
+        // plain loads: in a single launch every member loads at once and the
+        // wait at the barrier is short, so realigning after the arrival would
+        // add to the critical path (f32 4093 x 4099: 4151 -> 3600 GB/s)
+        gcs_body<T, BLOCK, JR, SP, false>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
+                                          g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds, ord);
         __syncthreads();  // LDS is reused by the next slot
     }
 }

@@ -262,3 +262,51 @@ def test_uneven_workload_full_4096(ipt, oracle):
     assert check(3) == 0
     print("[batch 4096] two graph replays exact", flush=True)
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_rows_off_the_line_grid(ipt, oracle):
+    """Cooperative instances whose rows (or bases) are off the 128-byte line
+    grid, where the persistent kernel loads whole lines and realigns them in
+    registers: row pitches at every 4- / 8-byte offset class, bases off by
+    1..31 elements, several slices (each slice's base has its own offset),
+    members that straddle bands, narrow slices (fewer than 8 tile rows), and
+    line-aligned neighbours; every instance exact after an eager run and two
+    graph replays."""
+    rng = np.random.default_rng(20261204)
+    shapes, offs = [], []
+    for k in range(40):
+        esz = 4 if k % 2 == 0 else 8
+        d1 = int(rng.integers(300, 3000))
+        d2 = int(rng.integers(40, 1600)) if k % 5 else int(rng.integers(33, 200))
+        d3 = int(rng.integers(1, 3))
+        if k % 7 == 0:  # line-aligned pitch, base off the grid
+            d1 = (d1 // (128 // esz) + 1) * (128 // esz)
+        shapes.append((d1, d2, d3, esz))
+        offs.append(int(rng.integers(0, 32)) if k % 3 else 0)
+    nbytes = sum((d1 * d2 * d3 * e + o * e + 255) // 256 * 256 for (d1, d2, d3, e), o in zip(shapes, offs))
+    buf = torch.empty(nbytes // 8 + 1, dtype=torch.int64, device="cuda")
+    raw = buf.view(torch.uint8)
+    items, host, pos = [], [], 0
+    for k, ((d1, d2, d3, esz), o) in enumerate(zip(shapes, offs)):
+        n = d1 * d2 * d3
+        a = oracle.random_bits(n, esz, 1500 + k)
+        start = pos + o * esz
+        t = raw[start:start + n * esz].view(torch.int32 if esz == 4 else torch.int64)
+        t.copy_(torch.from_numpy(a.view(np.int32 if esz == 4 else np.int64)))
+        items.append((t, d1, d2, d3, ipt.VARIANT_GCS))
+        host.append(a)
+        pos += (n * esz + o * esz + 255) // 256 * 256
+    b = ipt.Batch(items)
+    print("[batch off-grid]", b.info(), flush=True)
+    plain = [it[:4] for it in items]
+    b.run(graph=False)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, plain, host, 1)
+    b.run(graph=True)
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, plain, host, 3)
+    b.close()
