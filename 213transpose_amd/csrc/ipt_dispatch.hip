@@ -23,13 +23,28 @@
 namespace ipt {
 
 // ---------------------------------------------------------------------------
-// Plain copy (calibration reference for bandwidth and traffic counters):
-// 16 bytes per lane, grid-stride.  n16 = number of 16-byte words.
+// Plain copy (the bandwidth ceiling reported beside the transposes, and the
+// calibration of the traffic counters): each workgroup streams whole blocks
+// of bv 16-byte words, 8 per thread in flight, with non-temporal loads and
+// stores.  n16 = number of 16-byte words.  Measured on MI355X, 7.5 GB:
+// 5.38 TB/s with 256 KB blocks, against 4.54-4.79 TB/s for a grid-stride
+// copy of one word per thread (4.26 for hipMemcpy).
 __global__ void __launch_bounds__(256)
-k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+k_copy16(u32x4* __restrict__ dst, const u32x4* __restrict__ src, uint64_t n16, uint64_t bv) {
+    constexpr uint32_t U = 8;
+    const uint64_t nb = (n16 + bv - 1) / bv;
+    for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint64_t e = umin64(n16, (b + 1) * bv);
+        for (uint64_t i = b * bv + threadIdx.x; i < e; i += U * 256) {
+            u32x4 r[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                if (i + u * 256 < e) r[u] = __builtin_nontemporal_load(src + i + u * 256);
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                if (i + u * 256 < e) __builtin_nontemporal_store(r[u], dst + i + u * 256);
+        }
+    }
 }
 
 // Out-of-place transpose (the comparator; the reference times cuTT's
@@ -765,8 +780,12 @@ ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hi
     if (((uintptr_t)dst | (uintptr_t)src | bytes) % 16) return IPT_ERR_INVALID_ARG;
     const uint64_t n16 = bytes / 16;
     if (n16 == 0) return IPT_OK;
-    const int grid = (int)std::min<uint64_t>((n16 + 255) / 256, (uint64_t)ncu_of(ctx) * 16);
-    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, (uint4*)dst, (const uint4*)src, n16);
+    // blocks of 256 KB where the copy is large enough to give every one of
+    // 8 workgroups per CU several; smaller blocks (down to 16 KB) below that
+    const uint64_t slots = (uint64_t)ncu_of(ctx) * 8;
+    const uint64_t bv = std::max<uint64_t>(1024, std::min<uint64_t>(16384, (n16 / slots) / 2048 * 2048));
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((n16 + bv - 1) / bv, slots));
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, (u32x4*)dst, (const u32x4*)src, n16, bv);
     IPT_LAUNCH_CHECK("k_copy16");
     return IPT_OK;
 }
