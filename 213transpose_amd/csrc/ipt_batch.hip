@@ -56,8 +56,24 @@ k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items
                                      : 1 + (own + step - 2) % nq;
         const uint32_t lo = seg[q], hi = seg[q + 1];
         if (lo == hi) continue;
+        // ahead: a take made (thread 0) as soon as the current cooperative
+        // item's group released its members, so the atomic's round trip
+        // overlaps the item's stores.  The deadlock argument is unchanged: the
+        // workgroup takes its next item only once its current group has
+        // completed its barrier.
+        uint32_t ahead = 0;
+        bool have_ahead = false;
+        auto take_ahead = [&]() {
+            if (threadIdx.x == 0) {
+                ahead = lo + atomic_add_agent(&ctl[kQueueBase + q], 1u);
+                have_ahead = true;
+            }
+        };
         for (;;) {
-            if (threadIdx.x == 0) next = lo + atomic_add_agent(&ctl[kQueueBase + q], 1u);
+            if (threadIdx.x == 0) {
+                next = have_ahead ? ahead : lo + atomic_add_agent(&ctl[kQueueBase + q], 1u);
+                have_ahead = false;
+            }
             __syncthreads();
             const uint32_t s = next;
             __syncthreads();  // `next` is rewritten by the next take
@@ -65,9 +81,9 @@ k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items
             const BatchItem it = items[s];
             const BatchInst I = insts[it.inst];
             if (I.esz == 4)
-                batch_item<uint32_t>(I, it, ctl, bars, reinterpret_cast<uint32_t*>(lds));
+                batch_item<uint32_t>(I, it, ctl, bars, reinterpret_cast<uint32_t*>(lds), take_ahead);
             else
-                batch_item<uint64_t>(I, it, ctl, bars, reinterpret_cast<uint64_t*>(lds));
+                batch_item<uint64_t>(I, it, ctl, bars, reinterpret_cast<uint64_t*>(lds), take_ahead);
             __syncthreads();  // LDS is reused by the next item
         }
     }

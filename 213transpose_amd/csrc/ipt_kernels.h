@@ -1008,11 +1008,16 @@ inline TileOrder banded_order(uint32_t nw, uint32_t nti, uint32_t ntj) {
 // the group barrier, so the shuffling overlaps the wait for the other
 // members.  (Single launches keep plain loads: every member loads at once
 // there and the wait is short.)
-template <typename T, int BLOCK, int JR, int SP, bool AL = true>
+struct NoHook {
+    __device__ void operator()() const {}
+};
+// `released`: called by every thread once the group barrier has released
+// the member, before its stores.
+template <typename T, int BLOCK, int JR, int SP, bool AL = true, typename HOOK = NoHook>
 __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
                                           uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
                                           uint32_t m, uint32_t* bar, uint32_t* ctl, u32x4* lds,
-                                          const TileOrder& ord = TileOrder{}) {
+                                          const TileOrder& ord = TileOrder{}, HOOK released = HOOK{}) {
     using W = WaveTile<T>;
     constexpr uint32_t NW = BLOCK / 64;
     constexpr int JL = gcs_wave_lds_tiles(sizeof(T));
@@ -1135,6 +1140,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     }
     if (g > 1 && threadIdx.x == 0 && !group_wait(bar, g, wd.extra, wd.limit)) report_watchdog(ctl);
     __syncthreads();
+    released();
 #pragma unroll
     for (int j = 0; j < JL; j++) {
         const uint32_t k = wave + j * NW;
@@ -1221,9 +1227,9 @@ struct BatchItem {
 
 constexpr int kBatchBlock = 512;
 
-template <typename T>
+template <typename T, typename HOOK>
 __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& it, uint32_t* ctl,
-                                           uint32_t* bars, T* lds) {
+                                           uint32_t* bars, T* lds, HOOK released) {
     constexpr int TS_SWAP = sizeof(T) == 4 ? 64 : 32;
     T* base = reinterpret_cast<T*>(I.data) + it.slice * (uint64_t)I.R * I.C;
     switch (I.kind) {
@@ -1240,11 +1246,11 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
         u32x4* l = reinterpret_cast<u32x4*>(lds);
         static_assert(kBatchJrClass == 1, "one body per register-tile class");
         if (I.jr == 0)
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
-                                                              I.ntiles, it.sub, bar, ctl, l, I.ord);
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0), 0, true, HOOK>(
+                base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles, it.sub, bar, ctl, l, I.ord, released);
         else
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1), 0>(base, I.R, I.C, I.g, I.K, I.KL, I.dntj,
-                                                              I.ntiles, it.sub, bar, ctl, l, I.ord);
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1), 0, true, HOOK>(
+                base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles, it.sub, bar, ctl, l, I.ord, released);
         break;
     }
     case BK_SWAP_RS:  // buffer-resource form: slices below 2^31 bytes, any side
