@@ -158,13 +158,42 @@ k_slice_lds(T* __restrict__ data, uint32_t R, uint32_t C, uint32_t P, uint64_t n
 }
 
 // ---------------------------------------------------------------------------
-// Square slices N x N: swap-transpose tile pairs (I,J) / (J,I), I <= J.
+// Square slices N x N: swap-transpose tile pairs (I,J) / (J,I), I <= J, of an
+// nt x nt tile grid.  Pair t in column order: t = J(J+1)/2 + I, so the
+// workgroups running at one time take neighbouring tiles of a few tile
+// columns, whose (I,J) tiles lie TS rows apart in one column.  In diagonal
+// order (dord) the pairs with J - I = d are consecutive (d = 0, 1, ...; S(d) =
+// d*nt - d(d-1)/2 pairs come before diagonal d), so both tiles of
+// neighbouring pairs move along a row and a column at once.  Where the row
+// pitch is a multiple of 64 KB, the column-order tiles' row segments all fall
+// on the same few memory channels: f32 16384^2 4746 -> 4952 GB/s with the
+// diagonal order; 15360^2, 12288^2 and 8192^2 1-2 % slower with it.
+__device__ __forceinline__ void swap_pair(uint64_t t, uint64_t nt, bool dord, uint64_t* I, uint64_t* J) {
+    if (!dord) {
+        uint64_t j = (uint64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while (j * (j + 1) / 2 > t) j--;
+        while ((j + 1) * (j + 2) / 2 <= t) j++;
+        *I = t - j * (j + 1) / 2;
+        *J = j;
+        return;
+    }
+    const double b = 2.0 * (double)nt + 1.0;
+    const double disc = b * b - 8.0 * (double)t;
+    int64_t d = (int64_t)((b - sqrt(disc > 0.0 ? disc : 0.0)) * 0.5);
+    auto S = [nt](int64_t e) { return (uint64_t)(e * (int64_t)nt - e * (e - 1) / 2); };
+    if (d < 0) d = 0;
+    while (d > 0 && S(d) > t) d--;
+    while (S(d + 1) <= t) d++;
+    *I = t - S(d);
+    *J = *I + (uint64_t)d;
+}
+
 // One sweep; both tiles are staged in LDS (2 * TS * (TS+1) elements at lds)
-// before either is written.  Pair t = J(J+1)/2 + I.  VEC: 16-byte global
-// loads / stores (N*esz a multiple of 16 and a 16-byte aligned base); both
-// tiles' loads are in flight before the first LDS store.
+// before either is written.  VEC: 16-byte global loads / stores (N*esz a
+// multiple of 16 and a 16-byte aligned base); both tiles' loads are in flight
+// before the first LDS store.
 template <typename T, int TS, int BLOCK, bool VEC>
-__device__ __forceinline__ void swap_body(T* __restrict__ base, uint32_t N, uint64_t t, T* lds) {
+__device__ __forceinline__ void swap_body(T* __restrict__ base, uint32_t N, uint64_t t, T* lds, bool dord) {
     constexpr uint32_t TP = TS + 1;
     constexpr uint32_t PER = VEC ? 16 / sizeof(T) : 1;
     constexpr uint32_t VPR = TS / PER;           // vectors per tile row
@@ -173,10 +202,8 @@ __device__ __forceinline__ void swap_body(T* __restrict__ base, uint32_t N, uint
     using V = typename std::conditional<VEC, uint4, T>::type;
     T* A = lds;
     T* B = lds + TS * TP;
-    uint64_t J = (uint64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while (J * (J + 1) / 2 > t) J--;
-    while ((J + 1) * (J + 2) / 2 <= t) J++;
-    const uint64_t I = t - J * (J + 1) / 2;
+    uint64_t I, J;
+    swap_pair(t, (N + TS - 1) / TS, dord, &I, &J);
     const uint32_t r0 = (uint32_t)I * TS, c0 = (uint32_t)J * TS;
     const bool diag = I == J;
     V ra[UPT], rb[UPT];
@@ -230,12 +257,12 @@ constexpr int kSwapBlock = 256;
 
 template <typename T, int TS, bool VEC>
 __global__ void __launch_bounds__(kSwapBlock)
-k_square_swap(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3) {
+k_square_swap(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3, bool dord) {
     __shared__ T tiles[2 * TS * (TS + 1)];
     const uint64_t total = pairs * d3;
     for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const uint64_t sl = w / pairs;
-        swap_body<T, TS, kSwapBlock, VEC>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles);
+        swap_body<T, TS, kSwapBlock, VEC>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles, dord);
         __syncthreads();
     }
 }
@@ -812,17 +839,15 @@ struct WaveTile {
 // element up to the row end.  (Only the last tile column has them: I <= J,
 // so a pair whose row band is the last one is a diagonal pair.)
 template <typename T, int TS, int BLOCK>
-__device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, uint64_t t, T* lds) {
+__device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, uint64_t t, T* lds, bool dord) {
     constexpr uint32_t TP = TS + 1, PER = 16 / sizeof(T), VPR = TS / PER, VPT = TS * VPR;
     constexpr uint32_t UPT = (VPT + BLOCK - 1) / BLOCK;
     constexpr uint32_t E = sizeof(T);
     const Rsrc rs = make_rsrc(base, N * N * E);
     T* A = lds;
     T* B = lds + TS * TP;
-    uint64_t J = (uint64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while (J * (J + 1) / 2 > t) J--;
-    while ((J + 1) * (J + 2) / 2 <= t) J++;
-    const uint64_t I = t - J * (J + 1) / 2;
+    uint64_t I, J;
+    swap_pair(t, (N + TS - 1) / TS, dord, &I, &J);
     const uint32_t r0 = (uint32_t)I * TS, c0 = (uint32_t)J * TS;
     const bool diag = I == J;
     u32x4 ra[UPT], rb[UPT];
@@ -891,12 +916,12 @@ __device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, u
 
 template <typename T, int TS>
 __global__ void __launch_bounds__(kSwapBlock)
-k_square_swap_rs(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3) {
+k_square_swap_rs(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3, bool dord) {
     __shared__ T tiles[2 * TS * (TS + 1)];
     const uint64_t total = pairs * d3;
     for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const uint64_t sl = w / pairs;
-        swap_body_rs<T, TS, kSwapBlock>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles);
+        swap_body_rs<T, TS, kSwapBlock>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles, dord);
         __syncthreads();
     }
 }
@@ -1254,13 +1279,13 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
         break;
     }
     case BK_SWAP_RS:  // buffer-resource form: slices below 2^31 bytes, any side
-        swap_body_rs<T, TS_SWAP, kBatchBlock>(base, I.R, it.sub, lds);
+        swap_body_rs<T, TS_SWAP, kBatchBlock>(base, I.R, it.sub, lds, false);
         break;
     case BK_SWAP_VEC:
-        swap_body<T, TS_SWAP, kBatchBlock, true>(base, I.R, it.sub, lds);
+        swap_body<T, TS_SWAP, kBatchBlock, true>(base, I.R, it.sub, lds, false);
         break;
     default:
-        swap_body<T, TS_SWAP, kBatchBlock, false>(base, I.R, it.sub, lds);
+        swap_body<T, TS_SWAP, kBatchBlock, false>(base, I.R, it.sub, lds, false);
         break;
     }
 }

@@ -96,6 +96,43 @@ def test_square_swap_any_side_and_base(ipt, oracle, esz):
         assert np.array_equal(to_host(t, esz), exp), (n, d3, esz, off)
 
 
+def test_square_swap_diagonal_pair_order(ipt, oracle):
+    """Squares whose row pitch is a multiple of 64 KB take their tile pairs in
+    diagonal order: f64 8192^2 x 2 and f32 16384^2 off 16-byte alignment
+    through the buffer-resource form, f32 16384^2 with half-size tiles, all
+    against the oracle on random bits; f64 16384^2 (2 GiB, the 16-byte form)
+    against the closed form on iota input."""
+    import ctypes
+    L = ipt.lib()
+    for k, (n, d3, esz, off, tile) in enumerate([(8192, 2, 8, 0, 0), (16384, 1, 4, 1, 0),
+                                                  (16384, 1, 4, 0, 32)]):
+        print(f"[case] square n={n} d3={d3} esz={esz} off={off} tile={tile}", flush=True)
+        m = n * n * d3
+        a = oracle.random_bits(m + 4, esz, 900 + k)
+        t = to_dev(a)
+        view = t[off:off + m]
+        tune = ipt.Tune(tile=tile) if tile else None
+        st = L.ipt_transpose_variant(ctypes.c_void_p(view.data_ptr()), esz, n, n, d3,
+                                     ipt.VARIANT_SQUARE_SWAP, ctypes.byref(tune) if tune else None,
+                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        assert st == 0, (st, L.ipt_last_error())
+        torch.cuda.synchronize()
+        exp = a.copy()
+        exp[off:off + m] = oracle.seq_tt_213(a[off:off + m], n, n, d3)
+        assert np.array_equal(to_host(t, esz), exp), (n, d3, esz, off, tile)
+        del t, view, a, exp
+        torch.cuda.empty_cache()
+    n = 16384
+    print(f"[case] square n={n} esz=8 iota", flush=True)
+    t = iota_dev(n * n, 8)
+    ipt.transpose_variant(t, n, n, 1, ipt.VARIANT_SQUARE_SWAP)
+    torch.cuda.synchronize()
+    e = torch.arange(n * n, dtype=torch.int64, device="cuda").view(n, n).t().contiguous().view(-1)
+    assert torch.equal(t, e)
+    del t, e
+    torch.cuda.empty_cache()
+
+
 BLOCK_CYCLE_SHAPES = [  # (d1, d2, d3): skinny wide / tall, then large rectangles
     (65536, 32, 1), (32, 65536, 1), (7, 1 << 18, 2), (1 << 16, 5, 3), (128, 4096, 2),
     (4096, 128, 2), (3, 1 << 19, 1), (1 << 19, 3, 1), (2000, 64, 2), (64, 2000, 2),
