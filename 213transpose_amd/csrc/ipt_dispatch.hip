@@ -387,9 +387,10 @@ static ipt_status run_square_ts(DeviceCtx* ctx, T* data, const Shape& s, const T
     // sides keep the scalar form (2049^2 4690 vs 4389 through the resource).
     const bool rs = (vec || sizeof(T) == 4) && (uint64_t)N * N * sizeof(T) < ((uint64_t)1 << 31);
     auto k = rs ? k_square_swap_rs<T, TS> : vec ? k_square_swap<T, TS, true> : k_square_swap<T, TS, false>;
-    // Pairs in diagonal order where the row pitch is a multiple of 64 KB
-    // (swap_pair).
-    const bool dord = ((uint64_t)N * sizeof(T)) % 65536 == 0;
+    // Pairs in diagonal order where the row pitch is a power of two of 16 KB
+    // or more (swap_pair).
+    const uint64_t pitch = (uint64_t)N * sizeof(T);
+    const bool dord = pitch >= 16384 && (pitch & (pitch - 1)) == 0;
     int grid;
     grid_for(ctx, k, kSwapBlock, 0, pairs * (uint64_t)s.d3, rs ? 1 : t.grid_mult, &grid);
     hipLaunchKernelGGL(k, dim3(grid), dim3(kSwapBlock), 0, st, data, N, pairs, (uint64_t)s.d3, dord);

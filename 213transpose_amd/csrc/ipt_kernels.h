@@ -165,9 +165,11 @@ k_slice_lds(T* __restrict__ data, uint32_t R, uint32_t C, uint32_t P, uint64_t n
 // order (dord) the pairs with J - I = d are consecutive (d = 0, 1, ...; S(d) =
 // d*nt - d(d-1)/2 pairs come before diagonal d), so both tiles of
 // neighbouring pairs move along a row and a column at once.  Where the row
-// pitch is a multiple of 64 KB, the column-order tiles' row segments all fall
-// on the same few memory channels: f32 16384^2 4746 -> 4952 GB/s with the
-// diagonal order; 15360^2, 12288^2 and 8192^2 1-2 % slower with it.
+// pitch is a power of two of 16 KB or more, the column-order tiles' row
+// segments fall on a few memory channels (f32, GB/s, column -> diagonal:
+// 16384^2 4706 -> 4924, 8192^2 x 4 4755 -> 4985, 4096^2 x 16 4855 -> 4931);
+// other pitches are 0.5-1.5 % slower in diagonal order (2048^2 x 64,
+// 12288^2 x 2, 10240^2 x 3, 6144^2 x 8, 14336^2 x 2, 5120^2 x 12).
 __device__ __forceinline__ void swap_pair(uint64_t t, uint64_t nt, bool dord, uint64_t* I, uint64_t* J) {
     if (!dord) {
         uint64_t j = (uint64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);

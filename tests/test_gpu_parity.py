@@ -97,14 +97,15 @@ def test_square_swap_any_side_and_base(ipt, oracle, esz):
 
 
 def test_square_swap_diagonal_pair_order(ipt, oracle):
-    """Squares whose row pitch is a multiple of 64 KB take their tile pairs in
-    diagonal order: f64 8192^2 x 2 and f32 16384^2 off 16-byte alignment
-    through the buffer-resource form, f32 16384^2 with half-size tiles, all
-    against the oracle on random bits; f64 16384^2 (2 GiB, the 16-byte form)
-    against the closed form on iota input."""
+    """Squares whose row pitch is a power of two of 16 KB or more take their
+    tile pairs in diagonal order: f32 4096^2 x 2, f64 4096^2 and 8192^2 x 2,
+    f32 16384^2 off 16-byte alignment through the buffer-resource form, f32
+    16384^2 with half-size tiles, all against the oracle on random bits; f64
+    16384^2 (2 GiB, the 16-byte form) against the closed form on iota input."""
     import ctypes
     L = ipt.lib()
-    for k, (n, d3, esz, off, tile) in enumerate([(8192, 2, 8, 0, 0), (16384, 1, 4, 1, 0),
+    for k, (n, d3, esz, off, tile) in enumerate([(4096, 2, 4, 0, 0), (4096, 1, 8, 2, 0),
+                                                  (8192, 2, 8, 0, 0), (16384, 1, 4, 1, 0),
                                                   (16384, 1, 4, 0, 32)]):
         print(f"[case] square n={n} d3={d3} esz={esz} off={off} tile={tile}", flush=True)
         m = n * n * d3
