@@ -190,6 +190,8 @@ def lib():
     L.ipt_debug_chunk_moves.argtypes = [_i64, _i64, _i64, _int, _vp, _vp, _vp, _vp, _vp, _i64]
     L.ipt_transpose_oop.restype = _int
     L.ipt_transpose_oop.argtypes = [_vp, _vp, _int, _i64, _i64, _i64, _vp]
+    L.ipt_debug_gcs_deps.restype = _int
+    L.ipt_debug_gcs_deps.argtypes = [_int, _i64, _i64, ctypes.c_uint32, ctypes.c_uint32, _int, _vp, _vp]
     L.ipt_memory_kind.restype = _int
     L.ipt_memory_kind.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
     L.ipt_version.restype = ctypes.c_char_p
@@ -671,6 +673,18 @@ def chunk_moves(rc, cc, ell, device=True):
                                        hpos.ctypes.data, cap), "ipt_debug_chunk_moves")
     ni, nm, nh = (int(c) for c in counts)
     return items[:ni].copy(), msrc[:nm].copy(), mdst[:nm].copy(), hpos[:nh].copy()
+
+
+def gcs_deps(esz, d1, d2, g, K, banded=True):
+    """For a d2 x d1 slice held by g members of K tiles: the range of members
+    each member waits for at the group barrier, as two uint32 numpy arrays
+    (lo, hi); lo > hi for a member that owns no tile (test hook, host code)."""
+    import numpy as np
+    lo = np.zeros(g, dtype=np.uint32)
+    hi = np.zeros(g, dtype=np.uint32)
+    _check(lib().ipt_debug_gcs_deps(esz, d1, d2, g, K, 1 if banded else 0, lo.ctypes.data,
+                                    hi.ctypes.data), "ipt_debug_gcs_deps")
+    return lo, hi
 
 
 def version():

@@ -57,10 +57,13 @@ k_batch(const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items
         const uint32_t lo = seg[q], hi = seg[q + 1];
         if (lo == hi) continue;
         // ahead: a take made (thread 0) as soon as the current cooperative
-        // item's group released its members, so the atomic's round trip
-        // overlaps the item's stores.  The deadlock argument is unchanged: the
-        // workgroup takes its next item only once its current group has
-        // completed its barrier.
+        // item's barrier released the member, so the atomic's round trip
+        // overlaps the item's stores.  The deadlock argument is unchanged: a
+        // member is released once the members it waits for have arrived, and
+        // every member arrives before it waits; by the time a workgroup takes
+        // any later item, every slot of its current group has been taken
+        // (earlier in the same queue, or the queue is exhausted), so the
+        // members still waiting in that group wait only for taken members.
         uint32_t ahead = 0;
         bool have_ahead = false;
         auto take_ahead = [&]() {

@@ -795,3 +795,26 @@ ipt_status run_copy(DeviceCtx* ctx, void* dst, const void* src, size_t bytes, hi
 }
 
 }  // namespace ipt
+
+using namespace ipt;
+
+ipt_status ipt_debug_gcs_deps(int elem_bytes, int64_t d1, int64_t d2, uint32_t g, uint32_t K,
+                              int banded, uint32_t* lo, uint32_t* hi) {
+    if ((elem_bytes != 4 && elem_bytes != 8) || d1 <= 0 || d2 <= 0 || g == 0 || K == 0 || !lo || !hi)
+        return IPT_ERR_INVALID_ARG;
+    const uint64_t TS = elem_bytes == 4 ? 32 : 16;
+    if ((uint64_t)d1 * (uint64_t)d2 >= ((uint64_t)1 << 31)) return IPT_ERR_TOO_LARGE;
+    const uint32_t R = (uint32_t)d2, C = (uint32_t)d1;
+    const uint32_t nti = (uint32_t)((R + TS - 1) / TS), ntj = (uint32_t)((C + TS - 1) / TS);
+    const uint32_t ntiles = nti * ntj;
+    if ((uint64_t)g * K < ntiles || (uint64_t)g * K >= ((uint64_t)1 << 31)) return IPT_ERR_INVALID_ARG;
+    const TileOrder ord = banded ? banded_order(kGcsBlock / 64, nti, ntj) : TileOrder{};
+    for (uint32_t m = 0; m < g; m++) {
+        uint32_t nk;
+        if (banded) nk = m * K < ntiles ? std::min(K, ntiles - m * K) : 0u;
+        else nk = m < ntiles ? std::min(K, (ntiles - m + g - 1) / g) : 0u;
+        if (elem_bytes == 4) gcs_deps<32>(m, nk, K, ntiles, g, R, C, ntj, ord, &lo[m], &hi[m]);
+        else gcs_deps<16>(m, nk, K, ntiles, g, R, C, ntj, ord, &lo[m], &hi[m]);
+    }
+    return IPT_OK;
+}

@@ -550,15 +550,16 @@ namespace ipt {
 // one workgroup's LDS.  Slot s is member s % g of the group that owns slice
 // s / g; workgroup b takes slots b, b + grid, b + 2*grid, ... (static
 // round-robin, no global counter).  Each member stages its share of the
-// slice's tiles (LDS plus registers), the group meets at a per-slice
-// arrival counter, then every member writes its tiles transposed.  All reads
-// of a slice complete before any write to it and slices never overlap, so no
-// workgroup ever reads data another one wrote: the counters carry no data and
-// relaxed agent-scope atomics suffice.
+// slice's tiles (LDS plus registers), arrives at the slice's group barrier,
+// waits for the members whose staged tiles its stores may overwrite (the
+// whole group, or in the batch kernel the range gcs_deps gives), then writes
+// its tiles transposed.  Every element is read before any write to it and
+// slices never overlap, so no workgroup ever reads data another one wrote:
+// the barrier words carry no data and relaxed agent-scope atomics suffice.
 //
 // Sync state (device memory, zero at rest; each launch restores it):
 //   ctl[2] watchdog flag (sticky); ctl[6..7] the host watchdog word's address
-//   cnt[bar_words(g) * S ...] the barrier lines of slice S (see
+//   cnt[bar_words(g) * S ...] the barrier state of slice S (see
 //   group_arrive / group_wait); no two slices share a line.
 // The spin is bounded: after ~2^26 polls the member records ctl[2] and
 // proceeds, so a broken invariant shows up as an error, never as a hang.
@@ -567,7 +568,7 @@ namespace ipt {
 // ipt_debug_set_watchdog, to trip the watchdog on purpose.
 //
 // Deadlock freedom: the grid never exceeds the resident workgroups and
-// g <= grid.  A member of slice S waits only for the other members of S, and
+// g <= grid.  A member of slice S waits only for other members of S, and
 // each of those reaches S after finishing slots that are smaller than S's
 // first slot (s - grid < S*g because s - S*g < g <= grid); by induction on
 // slot order every group completes.
@@ -597,22 +598,38 @@ __device__ __forceinline__ bool spin_until(uint32_t* ctr, uint32_t target, uint3
 // Wait until every vector-memory load this wave issued has returned its data.
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// Group barrier of g members over kBarLines 128-byte lines (one slice).
-// Up to kBarLines - 1 members: one arrival counter (line 0, word 0), each
-// member adds 1 and waits for g.  More members: line 1 + q, word 0 counts
-// the arrivals of sub-group q (members m with m % nsub == q, nsub =
-// kBarLines - 1); the last arriver of a sub-group bumps line 0 word 0 and
-// members wait for it to reach nsub.  Either way line 0 word 16 counts
-// departures and the last member to depart zeroes every word used (zero at
-// rest).
+// Group barrier of the g members of one slice, in one of two forms (state:
+// bar_words(g) words, zero at rest; the last member to depart zeroes every
+// word it used).
+//
+// Counters (single launches, where every member starts at once): up to
+// kBarLines - 1 members, one arrival counter (line 0, word 0), each member
+// adds 1 and waits for g.  More members: line 1 + q, word 0 counts the
+// arrivals of sub-group q (members m with m % nsub == q, nsub = kBarLines -
+// 1); the last arriver of a sub-group bumps line 0 word 0 and members wait
+// for it to reach nsub.  Line 0 word 16 counts departures.
+//
+// Flags (the persistent batch kernel, whose members start whenever a
+// workgroup frees up): word 32 + m is member m's arrival flag, and a member
+// waits only for the members whose staged tiles its stores may overwrite
+// (gcs_deps: a contiguous range lo..hi), one wave polling up to 64 flags per
+// load; word 16 counts departures.  A member no longer waits for the last
+// of its whole group to be taken from the queue: uneven 4096-matrix batch,
+// k_batch 3.71 -> 3.36 ms.  In single launches, where all members wait at
+// once, the polling of many flags costs more than the wait it saves (f32
+// 3000 x 1000, 256 members: 1457 -> 1105 GB/s; 4099 x 4093 4142 -> 3716).
 constexpr uint32_t kBarLines = 17;
 
-// Words of barrier state per slice for a group of g.
-IPT_HD uint64_t bar_words(uint32_t g) { return 32ull * (1 + (g < kBarLines - 1 ? g : kBarLines - 1)); }
+// Words of barrier state per slice for a group of g (either form).
+IPT_HD uint64_t bar_words(uint32_t g) {
+    const uint64_t cnt = 32ull * (1 + (g < kBarLines - 1 ? g : kBarLines - 1));
+    const uint64_t flg = 32ull + ((uint64_t)g + 31) / 32 * 32;
+    return cnt > flg ? cnt : flg;
+}
 
-// The two halves of the barrier (thread 0 of a member): arrive, then wait
-// until every member has arrived (false on watchdog).  A member may do work
-// of its own in between, as long as it needs nothing from the others.
+// Counters: arrive, then wait until every member has arrived (thread 0;
+// false on watchdog).  A member may do work of its own in between, as long
+// as it needs nothing from the others.
 __device__ __forceinline__ void group_arrive(uint32_t* bar, uint32_t m, uint32_t g) {
     if (g <= kBarLines - 1) {
         atomic_add_agent(&bar[0], 1u);
@@ -637,6 +654,37 @@ __device__ __forceinline__ void group_depart(uint32_t* bar, uint32_t g) {
     }
 }
 
+// Flags: arrive (thread 0), then wait (wave 0, every lane) until members
+// lo..hi have arrived (false on watchdog).
+__device__ __forceinline__ void flag_arrive(uint32_t* bar, uint32_t m) { atomic_store_agent(&bar[32 + m], 1u); }
+
+__device__ __forceinline__ bool flag_wait(uint32_t* bar, uint32_t lo, uint32_t hi, uint32_t extra,
+                                          uint32_t limit, uint32_t lane) {
+    uint32_t polls = 0;
+    for (uint32_t b = lo; b <= hi; b += 64) {
+        const uint32_t i = b + lane;
+        bool seen = i > hi;
+        for (;;) {
+            if (!seen) seen = atomic_load_agent(&bar[32 + i]) != 0;
+            if (__all(seen)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (++polls > limit) return false;
+        }
+    }
+    if (extra) {  // test hook: wait for arrivals that never come
+        while (++polls <= limit) __builtin_amdgcn_s_sleep(1);
+        return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void flag_depart(uint32_t* bar, uint32_t g) {
+    if (atomic_add_agent(&bar[16], 1u) == g - 1) {
+        for (uint32_t m = 0; m < g; m++) atomic_store_agent(&bar[32 + m], 0u);
+        atomic_store_agent(&bar[16], 0u);
+    }
+}
+
 // Staging shared by the members of one slice: after every wave has its
 // loads back, thread 0 arrives, and before the first store it waits for
 // the group (group_arrive / group_wait); `bar` is null when the slice has
@@ -647,7 +695,7 @@ __device__ __forceinline__ void group_depart(uint32_t* bar, uint32_t g) {
 struct Watchdog {
     uint32_t limit = kSpinLimit, extra = 0;
     __device__ __forceinline__ void read(const uint32_t* ctl, uint32_t g) {
-        if (g > 1 && threadIdx.x == 0) {
+        if (g > 1 && threadIdx.x < 64) {
             const uint32_t l = ctl[3];
             limit = l ? l : kSpinLimit;
             extra = ctl[4];
@@ -1017,6 +1065,48 @@ inline TileOrder banded_order(uint32_t nw, uint32_t nti, uint32_t ntj) {
     return o;
 }
 
+// The members whose staged tiles member m's stores may overwrite, lo..hi
+// (empty when m owns no tile).  Strided order: the whole group.  Banded
+// order: m's tiles lie in tile columns tja..tjb (those of one band; every
+// column when its positions run into the next band), so it writes only
+// output rows tja*TS .. (tjb+1)*TS - 1 of R elements each, slice elements
+// e0 .. e1 - 1, which hold input rows e0 / C .. (e1 - 1) / C; every member
+// owning a position in those rows' bands is waited for.
+template <int TS>
+IPT_HD void gcs_deps(uint32_t m, uint32_t nk, uint32_t K, uint32_t ntiles, uint32_t g, uint32_t R,
+                     uint32_t C, uint32_t ntj, const TileOrder& ord, uint32_t* lo, uint32_t* hi) {
+    if (ord.bh == 0) {
+        *lo = 0;
+        *hi = g - 1;
+        return;
+    }
+    if (nk == 0) {
+        *lo = 1;
+        *hi = 0;
+        return;
+    }
+    const uint32_t bandpos = ord.dband.d;  // bh * ntj positions per full band
+    auto band = [&](uint32_t p) {
+        const uint32_t b = ord.dband.div(p);
+        return b < ord.full ? b : ord.full;
+    };
+    const uint32_t p0 = m * K, p1 = m * K + nk - 1;
+    const uint32_t b0 = band(p0), b1 = band(p1);
+    uint32_t tja = 0, tjb = ntj - 1;
+    if (b0 == b1) {
+        const Div32& dh = b0 < ord.full ? ord.dbh : ord.dlast;
+        tja = dh.div(p0 - b0 * bandpos);
+        tjb = dh.div(p1 - b0 * bandpos);
+    }
+    const uint32_t rows_end = (tjb + 1) * (uint32_t)TS < C ? (tjb + 1) * (uint32_t)TS : C;
+    const uint64_t e0 = (uint64_t)tja * TS * R, e1 = (uint64_t)rows_end * R;
+    const uint32_t tr0 = (uint32_t)(e0 / C) / TS, tr1 = (uint32_t)((e1 - 1) / C) / TS;
+    const uint32_t B0 = band(tr0 * ntj), B1 = band(tr1 * ntj);  // the bands of tile rows tr0, tr1
+    const uint32_t q1 = B1 < ord.full ? (B1 + 1) * bandpos : ntiles;
+    *lo = B0 * bandpos / K;
+    *hi = (q1 - 1) / K < g - 1 ? (q1 - 1) / K : g - 1;
+}
+
 // One GCS slot: member m of the group of g that transposes the
 // slice at `base`.  Member m owns tiles t = m + k*g, k < nk; wave w of the
 // workgroup takes k = w, w + NW, ...  Tiles k < kl are staged in LDS
@@ -1040,7 +1130,7 @@ struct NoHook {
 };
 // `released`: called by every thread once the group barrier has released
 // the member, before its stores.
-template <typename T, int BLOCK, int JR, int SP, bool AL = true, typename HOOK = NoHook>
+template <typename T, int BLOCK, int JR, int SP, bool AL = true, bool DEP = true, typename HOOK = NoHook>
 __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint32_t C, uint32_t g,
                                           uint32_t K, uint32_t KL, Div32 dntj, uint32_t ntiles,
                                           uint32_t m, uint32_t* bar, uint32_t* ctl, u32x4* lds,
@@ -1136,7 +1226,10 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     if (sw) {
         wait_vmem();
         __syncthreads();
-        if (g > 1 && threadIdx.x == 0) group_arrive(bar, m, g);
+        if (g > 1 && threadIdx.x == 0) {
+            if constexpr (DEP) flag_arrive(bar, m);
+            else group_arrive(bar, m, g);
+        }
     }
 #pragma unroll
     for (int j = 0; j < JL; j++) {
@@ -1163,9 +1256,20 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     if (!sw) {
         wait_vmem();
         __syncthreads();
-        if (g > 1 && threadIdx.x == 0) group_arrive(bar, m, g);
+        if (g > 1 && threadIdx.x == 0) {
+            if constexpr (DEP) flag_arrive(bar, m);
+            else group_arrive(bar, m, g);
+        }
     }
-    if (g > 1 && threadIdx.x == 0 && !group_wait(bar, g, wd.extra, wd.limit)) report_watchdog(ctl);
+    if constexpr (DEP) {
+        if (g > 1 && threadIdx.x < 64) {
+            uint32_t lo, hi;
+            gcs_deps<W::TS>(m, nk, K, ntiles, g, R, C, dntj.d, ord, &lo, &hi);
+            if (!flag_wait(bar, lo, hi, wd.extra, wd.limit, lane) && lane == 0) report_watchdog(ctl);
+        }
+    } else {
+        if (g > 1 && threadIdx.x == 0 && !group_wait(bar, g, wd.extra, wd.limit)) report_watchdog(ctl);
+    }
     __syncthreads();
     released();
 #pragma unroll
@@ -1187,7 +1291,10 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
         origin(k, &r0, &c0);
         rt[j].template store_t<SP>(rs, r0, c0, R, C, lane, k < nk);
     }
-    if (g > 1 && threadIdx.x == 0) group_depart(bar, g);
+    if (g > 1 && threadIdx.x == 0) {
+        if constexpr (DEP) flag_depart(bar, g);
+        else group_depart(bar, g);
+    }
 }
 
 // GCS kernel: slot s (numbered by xcd_slot) is member s % g of
@@ -1204,10 +1311,12 @@ k_gcs(T* __restrict__ data, uint32_t R, uint32_t C, uint64_t nslices, uint32_t g
     for (uint64_t slot = xcd_slot(blockIdx.x, gridDim.x, nxcd, g); slot < total; slot += gridDim.x) {
         const uint64_t S = slot / g;
         const uint32_t m = (uint32_t)(slot - S * g);
-        // plain loads: in a single launch every member loads at once and the
-        // wait at the barrier is short, so realigning after the arrival would
-        // add to the critical path (f32 4093 x 4099: 4151 -> 3600 GB/s)
-        gcs_body<T, BLOCK, JR, SP, false>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
+        // plain loads and the counter barrier: in a single launch every
+        // member loads at once and the wait at the barrier is short, so
+        // realigning after the arrival would add to the critical path (f32
+        // 4093 x 4099: 4151 -> 3600 GB/s), and polling per-member flags
+        // costs more than it saves
+        gcs_body<T, BLOCK, JR, SP, false, false>(data + S * slice_elems, R, C, g, K, KL, dntj, ntiles, m,
                                           g > 1 ? cnt + bar_words(g) * S : nullptr, ctl, lds, ord);
         __syncthreads();  // LDS is reused by the next slot
     }
@@ -1273,10 +1382,10 @@ __device__ __forceinline__ void batch_item(const BatchInst& I, const BatchItem& 
         u32x4* l = reinterpret_cast<u32x4*>(lds);
         static_assert(kBatchJrClass == 1, "one body per register-tile class");
         if (I.jr == 0)
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0), 0, true, HOOK>(
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 0), 0, true, true, HOOK>(
                 base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles, it.sub, bar, ctl, l, I.ord, released);
         else
-            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1), 0, true, HOOK>(
+            gcs_body<T, kBatchBlock, gcs_jr_of(sizeof(T), 1), 0, true, true, HOOK>(
                 base, I.R, I.C, I.g, I.K, I.KL, I.dntj, I.ntiles, it.sub, bar, ctl, l, I.ord, released);
         break;
     }

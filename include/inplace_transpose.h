@@ -303,6 +303,17 @@ ipt_status ipt_debug_chunk_moves(int64_t rc, int64_t cc, int64_t ell, int builde
                                  int64_t counts[3], uint32_t* items, uint32_t* msrc,
                                  uint32_t* mdst, uint32_t* hpos, int64_t cap);
 
+/* Test hook for the cooperative kernels' group barrier: for a d2 x d1 slice
+ * of elem_bytes elements held by g members of K tiles each, in the banded
+ * tile order (banded != 0, bands as tall as a workgroup's 8 waves) or the
+ * strided one, writes for every member m < g the range lo[m]..hi[m] of
+ * members it waits for before its stores (those whose staged tiles its
+ * stores may overwrite; lo > hi when it owns no tile).  Pure host code (the
+ * function the kernels run).  No counterpart in the reference, whose kernels
+ * synchronise the whole grid. */
+ipt_status ipt_debug_gcs_deps(int elem_bytes, int64_t d1, int64_t d2, uint32_t g, uint32_t K,
+                              int banded, uint32_t* lo, uint32_t* hi);
+
 /* ---- autotuner and learned selector ----------------------------------- */
 /* No counterpart in the reference: its tier thresholds are magic numbers
  * (src/shuffle.cu:15-50, src/rotate.cu:241-258).  Here IPT_NUM_CANDIDATES

@@ -152,3 +152,74 @@ def test_views_without_exact_64bit_plan_are_refused(ipt):
     assert rc == 3  # IPT_ERR_TOO_LARGE
     ok = (1 << 31) + 11  # b^2 < 2^63: the 64-bit plan is exact
     assert ipt.plan_query(4, ok, 3)["index_bits"] == 64
+
+
+def _gcs_owners(esz, d1, d2, g, K, banded):
+    """Owner member of every tile of a d2 x d1 slice (nti x ntj), restated
+    from the tile orders' definitions: banded, positions walked band by band
+    (bands 8 tile rows tall, the last one shorter), column by column inside a
+    band, member m owning positions m*K .. m*K + K - 1; strided, tile t = ti *
+    ntj + tj owned by member t % g."""
+    TS = 32 if esz == 4 else 16
+    nti, ntj = -(-d2 // TS), -(-d1 // TS)
+    owner = np.empty((nti, ntj), dtype=np.int64)
+    if not banded:
+        owner[:] = (np.arange(nti * ntj) % g).reshape(nti, ntj)
+        return owner, TS
+    bh = min(8, nti)
+    p = 0
+    for b0 in range(0, nti, bh):
+        h = min(bh, nti - b0)
+        for tj in range(ntj):
+            for r in range(h):
+                owner[b0 + r, tj] = p // K
+                p += 1
+    return owner, TS
+
+
+def _gcs_dep_violations(esz, d1, d2, g, K, banded, lo, hi):
+    """Elements whose loader is neither their writer nor in the writer's
+    wait range lo..hi (brute force over every element of the slice)."""
+    owner, TS = _gcs_owners(esz, d1, d2, g, K, banded)
+    R, C = d2, d1
+    e = np.arange(R * C, dtype=np.int64)
+    loader = owner[(e // C) // TS, (e % C) // TS]        # holds input element e
+    writer = owner[(e % R) // TS, (e // R) // TS]        # stores output element e
+    lo64, hi64 = lo.astype(np.int64), hi.astype(np.int64)
+    ok = (loader == writer) | ((loader >= lo64[writer]) & (loader <= hi64[writer]))
+    return int((~ok).sum())
+
+
+def test_gcs_barrier_waits_cover_every_overwritten_tile(ipt):
+    """The cooperative kernels' group barrier lets a member store once the
+    members in its range lo..hi have arrived: every member whose staged tiles
+    hold an element it overwrites must be in that range (brute force over
+    every element, tile owners restated from the orders' definitions), on
+    random slices, both element sizes, both tile orders, with and without
+    idle members.  The banded ranges must also be narrower than the group
+    somewhere, and a range cut short by one member must be caught."""
+    rng = np.random.default_rng(5)
+    narrowed = caught = 0
+    for case in range(160):
+        esz = 4 if case % 2 else 8
+        TS = 32 if esz == 4 else 16
+        d1 = int(rng.integers(1, 900)) if case % 5 else int(rng.integers(1, 40))
+        d2 = int(rng.integers(1, 900)) if case % 7 else int(rng.integers(1, 40))
+        ntiles = (-(-d2 // TS)) * (-(-d1 // TS))
+        K = int(rng.integers(1, max(2, ntiles // 3) + 1))
+        g = -(-ntiles // K) + int(rng.integers(0, 3))
+        for banded in (True, False):
+            lo, hi = ipt.gcs_deps(esz, d1, d2, g, K, banded)
+            assert _gcs_dep_violations(esz, d1, d2, g, K, banded, lo, hi) == 0, (esz, d1, d2, g, K, banded)
+            live = lo <= hi
+            if banded and live.any() and (hi[live] - lo[live] + 1).max() < g:
+                narrowed += 1
+            # a planted mistake: one member's range cut short at its top
+            cut = np.where(live & (hi > lo))[0]
+            if len(cut):
+                m = int(cut[rng.integers(0, len(cut))])
+                hi2 = hi.copy()
+                hi2[m] -= 1
+                caught += _gcs_dep_violations(esz, d1, d2, g, K, banded, lo, hi2) > 0
+    assert narrowed >= 20, narrowed
+    assert caught >= 40, caught
