@@ -147,6 +147,8 @@ def lib():
     L.ipt_batch_get_routes.argtypes = [_vp, _vp]
     L.ipt_batch_check.restype = _int
     L.ipt_batch_check.argtypes = [_vp]
+    L.ipt_batch_debug_set_watchdog.restype = _int
+    L.ipt_batch_debug_set_watchdog.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
     L.ipt_batch_destroy.restype = _int
     L.ipt_batch_destroy.argtypes = [_vp]
     L.ipt_candidate.restype = _int
@@ -459,6 +461,12 @@ class Batch:
     def check(self):
         """Synchronise and raise if a group barrier of this batch gave up waiting."""
         _check(lib().ipt_batch_check(self._h), "ipt_batch_check")
+
+    def set_watchdog(self, polls, extra):
+        """Test hook: later runs' group barriers give up after `polls` polls and
+        wait for `extra` arrivals that never come (0, 0: normal)."""
+        _check(lib().ipt_batch_debug_set_watchdog(self._h, polls, extra),
+               "ipt_batch_debug_set_watchdog")
 
     def close(self):
         if self._h:

@@ -737,6 +737,20 @@ ipt_status ipt_batch_check(ipt_batch* b) {
     return IPT_OK;
 }
 
+ipt_status ipt_batch_debug_set_watchdog(ipt_batch* b, uint32_t polls, uint32_t extra) {
+    if (!b) return IPT_ERR_INVALID_ARG;
+    const uint32_t w[2] = {polls, extra};
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(b->d_sync + 3, w, sizeof(w), hipMemcpyHostToDevice);
+    if (e == hipSuccess && b->big_sync.ptr)
+        e = hipMemcpy((uint32_t*)b->big_sync.ptr + 3, w, sizeof(w), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error("ipt_batch_debug_set_watchdog", e);
+        return IPT_ERR_HIP;
+    }
+    return IPT_OK;
+}
+
 ipt_status ipt_batch_destroy(ipt_batch* b) {
     if (!b) return IPT_ERR_INVALID_ARG;
     (void)hipDeviceSynchronize();

@@ -459,6 +459,13 @@ ipt_status ipt_batch_kernel_ms(ipt_batch* b, float* ms);
 /* Synchronises the device and reports (IPT_ERR_HIP) whether a group barrier
  * of this batch gave up waiting; the barrier state is then reset. */
 ipt_status ipt_batch_check(ipt_batch* b);
+/* Test hook for this batch's group barriers (as ipt_debug_set_watchdog for
+ * standalone launches): in later runs every member gives up after `polls`
+ * polls (0: the default, ~2^26) and also waits for `extra` arrivals that
+ * never come (0 in use), so a test can trip the watchdog without a real
+ * fault; ipt_batch_check then reports it and clears both settings.
+ * Synchronises the device. */
+ipt_status ipt_batch_debug_set_watchdog(ipt_batch* b, uint32_t polls, uint32_t extra);
 /* Synchronises the device, then frees the batch. */
 ipt_status ipt_batch_destroy(ipt_batch* b);
 

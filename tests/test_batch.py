@@ -310,3 +310,40 @@ def test_batch_rows_off_the_line_grid(ipt, oracle):
     b.check()
     check(oracle, plain, host, 3)
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_barrier_watchdog_reports_and_recovers(ipt, oracle):
+    """The batch kernel's group barriers (per-member flags and dependency
+    ranges) give up through the watchdog hook (every member waits for an
+    arrival that never comes for 2000 polls, then proceeds, so every kernel
+    exits normally): ipt_batch_check reports it and restores the barrier
+    state, and the next run is exact again."""
+    shapes = [(1500, 1000, 1, 4), (777, 1023, 1, 8), (300, 2000, 2, 4), (40, 50, 3, 4)]
+    buf, items, host = make_instances(oracle, shapes, 1300)
+    b = ipt.Batch(items)
+    info = b.info()
+    print("[batch]", info, flush=True)
+    assert info["n_gcs_items"] > 0
+    b.set_watchdog(2000, 1)
+    print("[case] tripping the batch barriers on purpose", flush=True)
+    b.run(graph=False)
+    with pytest.raises(ipt.TransposeError) as err:
+        b.check()
+    assert "watchdog" in str(err.value)
+    b.check()  # state restored, settings cleared
+    buf2, items2, host2 = make_instances(oracle, shapes, 1400)
+    b2 = ipt.Batch(items2)
+    b2.run(graph=False)
+    torch.cuda.synchronize()
+    b2.check()
+    check(oracle, items2, host2, 1)
+    # the tripped batch itself runs normally again on fresh data
+    for (t, d1, d2, d3), a in zip(items, host):
+        t.copy_(torch.from_numpy(a.view(np.int32 if a.itemsize == 4 else np.int64)).cuda())
+    b.run(graph=False)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, items, host, 1)
+    b.close()
+    b2.close()
