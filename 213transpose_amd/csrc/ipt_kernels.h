@@ -1007,18 +1007,18 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n, uint32_t nx
 
 // Wave tiles a wave stages in LDS (JL) and holds in registers (JR) on the
 // cooperative kernel, per element size: 40 / 80 LDS tiles fill 160 KB, and the
-// register tiles hold 128 KB per 512-thread workgroup.
+// largest register-tile class holds 128 / 144 KB per 512-thread workgroup.
 constexpr int gcs_wave_lds_tiles(int esz) { return esz == 4 ? 5 : 10; }
-constexpr int gcs_wave_reg_tiles(int esz) { return esz == 4 ? 8 : 16; }
 constexpr int kBatchJrClass = 1;
-// Register tiles per wave are a template argument: JR = 1, JR_max / 2 or
-// JR_max, the smallest that holds a member's tiles beyond the LDS ones
+// Register tiles per wave are a template argument: JR = 1, 4 / 8 or 8 / 18
+// (f32 / f64), the smallest that holds a member's tiles beyond the LDS ones
 // (unused register tiles still cost load / store instructions and
-// registers).  gcs_jr_class picks 0, 1 or 2 for those three.  JR_max
-// (235 / 228 VGPRs) fits the two waves per SIMD that LDS allows; the batch
+// registers).  gcs_jr_class picks 0, 1 or 2 for those three.  The largest
+// (f32 235 VGPRs; f64 18 tiles, so that a slice of 4 x 256 x 222 tiles, 116 MB,
+// fits one group) fits the two waves per SIMD that LDS allows; the batch
 // kernel, which also holds the other item kinds, stops at class 1.
 constexpr int gcs_jr_of(int esz, int cls) {
-    return cls == 0 ? 1 : cls == 1 ? gcs_wave_reg_tiles(esz) / 2 : gcs_wave_reg_tiles(esz);
+    return cls == 0 ? 1 : cls == 1 ? (esz == 4 ? 4 : 8) : (esz == 4 ? 8 : 18);
 }
 // tail: classes 0 and 1 keep one register tile per wave free (gcs_geom).
 inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw, bool tail = false) {

@@ -128,13 +128,14 @@ def test_batch_block_cycle_instances(ipt, oracle):
 @pytest.mark.gpu
 def test_batch_heuristic_routes_oversized_instances(ipt, oracle):
     """With the selector off, instances too large for one cooperative group
-    of the persistent kernel: f64 3970 x 2975 (94 MB) fits one group of the
-    per-call kernel and runs as one separate cooperative launch; f64 3751 x
-    3723 (112 MB) fits neither and takes the block-cycle route with its
-    blocks in the batch; neither falls to a separate C2R / R2C launch.
-    Beside ordinary instances, every instance exact after an eager run and a
-    graph replay."""
-    shapes = [(3751, 3723, 1, 8), (3970, 2975, 1, 8), (512, 384, 2, 4), (33, 17, 4, 8)]
+    of the persistent kernel: f64 3970 x 2975 (94 MB) and 3751 x 3723 (112
+    MB) fit one group of the per-call kernel and run as separate cooperative
+    launches; f64 3900 x 4096 (128 MB) fits neither and takes the block-cycle
+    route with its blocks in the batch; none falls to a separate C2R / R2C
+    launch.  Beside ordinary instances, every instance exact after an eager
+    run and a graph replay."""
+    shapes = [(3900, 4096, 1, 8), (3751, 3723, 1, 8), (3970, 2975, 1, 8), (512, 384, 2, 4),
+              (33, 17, 4, 8)]
     buf, items, host = make_instances(oracle, shapes, 1300)
     ipt.selector_load(None)
     try:
@@ -143,7 +144,7 @@ def test_batch_heuristic_routes_oversized_instances(ipt, oracle):
         ipt.load_default_selector()
     routes = list(b.routes())
     print("[batch heuristic oversized]", b.info(), routes, flush=True)
-    assert routes[:2] == [ipt.ROUTE_BLOCK_CYCLE, ipt.ROUTE_GCS_LAUNCH]
+    assert routes[:3] == [ipt.ROUTE_BLOCK_CYCLE, ipt.ROUTE_GCS_LAUNCH, ipt.ROUTE_GCS_LAUNCH]
     assert ipt.ROUTE_SEPARATE not in routes
     b.run(graph=False)
     torch.cuda.synchronize()
