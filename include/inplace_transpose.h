@@ -188,7 +188,7 @@ typedef enum {
     IPT_VARIANT_GCS = 5,         /* group-cooperative staged: a group of workgroups
                                     holds a slice in LDS and registers, then
                                     writes it transposed; one sweep for slices
-                                    up to ~109 MB, any row length and element
+                                    up to ~109 MB (f32) or ~117 MB (f64), any row length and element
                                     alignment */
     IPT_VARIANT_BLOCK_CYCLE = 6, /* one side cut into chunks of T elements:
                                     the chunks moved by segmented cycle
