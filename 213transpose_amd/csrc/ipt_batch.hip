@@ -183,7 +183,7 @@ Route plan_instance(const DeviceCtx* ctx, const Shape& s, int forced, int64_t ld
     // C2R / R2C as separate launches.
     // (uneven 4096-matrix batch: three f64 instances of 88-96 MB as separate
     // launches instead of folded block-cycle, graph 4.23 -> 4.08 ms)
-    *big_variant = gcs_geom(ctx, s, lds_max_of(ctx), true, 2, true).ok ? IPT_VARIANT_GCS
+    *big_variant = gcs_geom(ctx, s, lds_max_of(ctx), true, 3, true).ok ? IPT_VARIANT_GCS
                                                                        : IPT_VARIANT_BLOCK_CYCLE;
     (void)t;
     return Route::Big;
@@ -523,7 +523,7 @@ ipt_status ipt_batch_create(const ipt_matrix_desc* descs, int64_t count, const i
             return IPT_ERR_UNSUPPORTED;
         }
         if (big_variant == IPT_VARIANT_GCS) {
-            const GcsGeom gg = gcs_geom(ctx, s, big_tune.lds_cap, true, 2, true);
+            const GcsGeom gg = gcs_geom(ctx, s, big_tune.lds_cap, true, 3, true);
             if (!gg.ok) {
                 release(b);
                 return IPT_ERR_UNSUPPORTED;

@@ -217,7 +217,10 @@ GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spr
     tail = tail && rows_off_lines(s);
     if (jr_cls <= 1) return gcs_geom_cls(ctx, s, lds_cap, spread, jr_cls, pack, tail);
     const GcsGeom small = gcs_geom_cls(ctx, s, lds_cap, spread, 1, pack, tail);
-    if (!small.ok) return gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack, false);
+    if (!small.ok) {
+        const GcsGeom c2 = gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack, false);
+        return c2.ok || jr_cls < 3 ? c2 : gcs_geom_cls(ctx, s, lds_cap, spread, 3, pack, false);
+    }
     if (!pack || small.grid == 0 || small.grid / small.g != 1) return small;
     const GcsGeom big = gcs_geom_cls(ctx, s, lds_cap, spread, 2, pack, false);
     return big.ok && big.grid && big.grid / big.g >= 2 ? big : small;
@@ -465,7 +468,7 @@ static ipt_status launch_gcs(DeviceCtx* ctx, GcsKernel<T> k, T* d, const Shape& 
 
 template <typename T>
 static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t, hipStream_t st) {
-    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, 2, true);
+    const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, 3, true);
     if (!gg.ok) return IPT_ERR_UNSUPPORTED;
     Arena& a = t.sync ? *t.sync : ctx->sync;
     // Banded tile ownership and write-back stores (the L2 merges the partial
@@ -492,13 +495,15 @@ static ipt_status run_gcs(DeviceCtx* ctx, T* data, const Shape& s, const Tune& t
         switch (cls) {
         case 0: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 0), 0>, data, s, gg, a, ord, st);
         case 1: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 1), 0>, data, s, gg, a, ord, st);
-        default: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 2), 0>, data, s, gg, a, ord, st);
+        case 2: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 2), 0>, data, s, gg, a, ord, st);
+        default: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 3), 0>, data, s, gg, a, ord, st);
         }
     }
     switch (cls) {
     case 0: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 0), kStoreDevice>, data, s, gg, a, ord, st);
     case 1: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 1), kStoreDevice>, data, s, gg, a, ord, st);
-    default: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 2), kStoreDevice>, data, s, gg, a, ord, st);
+    case 2: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 2), kStoreDevice>, data, s, gg, a, ord, st);
+    default: return launch_gcs<T>(ctx, k_gcs<T, kGcsBlock, gcs_jr_of(sizeof(T), 3), kStoreDevice>, data, s, gg, a, ord, st);
     }
 }
 

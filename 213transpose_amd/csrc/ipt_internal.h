@@ -175,7 +175,7 @@ struct GcsGeom {
 // (the line after a wave's run of tiles in the line-aligned loads of
 // gcs_body); classes below the largest only.
 GcsGeom gcs_geom(const DeviceCtx* ctx, const Shape& s, int64_t lds_cap, bool spread = true,
-                 int jr_cls = 2, bool pack = false, bool tail = false);
+                 int jr_cls = 3, bool pack = false, bool tail = false);
 // Whether rows of this shape are off the 128-byte line grid (the line-aligned
 // loads then want the tail register tile).
 inline bool rows_off_lines(const Shape& s) { return ((s.d1 * s.esz) & 127) != 0; }

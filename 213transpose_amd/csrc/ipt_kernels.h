@@ -1007,25 +1007,30 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n, uint32_t nx
 
 // Wave tiles a wave stages in LDS (JL) and holds in registers (JR) on the
 // cooperative kernel, per element size: 40 / 80 LDS tiles fill 160 KB, and the
-// largest register-tile class holds 128 / 144 KB per 512-thread workgroup.
+// register tiles hold up to 144 KB per 512-thread workgroup.
 constexpr int gcs_wave_lds_tiles(int esz) { return esz == 4 ? 5 : 10; }
 constexpr int kBatchJrClass = 1;
-// Register tiles per wave are a template argument: JR = 1, 4 / 8 or 8 / 18
-// (f32 / f64), the smallest that holds a member's tiles beyond the LDS ones
-// (unused register tiles still cost load / store instructions and
-// registers).  gcs_jr_class picks 0, 1 or 2 for those three.  The largest
-// (f32 235 VGPRs; f64 18 tiles, so that a slice of 4 x 256 x 222 tiles, 116 MB,
-// fits one group) fits the two waves per SIMD that LDS allows; the batch
-// kernel, which also holds the other item kinds, stops at class 1.
+// Register tiles per wave are a template argument: class 0, 1, 2 or 3 holds
+// 1, 4, 8 or 9 tiles (f32) and 1, 8, 16 or 18 (f64): the smallest class that
+// holds a member's tiles beyond the LDS ones (unused register tiles still
+// cost load / store instructions and registers).  Class 3 (256 / 250
+// VGPRs, no spills) only raises the capacity for slices that class 2
+// cannot hold in one group of 256 members: up to 116 MB instead of 109 /
+// 106 MB (f32 5400 x 5300: 2342 GB/s on the block-cycle path, 5080 in one
+// sweep; f64 3751 x 3723: 2239 -> 4927); as the class of every slice past
+// class 1 it cost up to 3.6 % (f32 2326 x 4023 x 28).  All fit the two waves
+// per SIMD that LDS allows; the batch kernel, which also holds the other
+// item kinds, stops at class 1.
 constexpr int gcs_jr_of(int esz, int cls) {
-    return cls == 0 ? 1 : cls == 1 ? (esz == 4 ? 4 : 8) : (esz == 4 ? 8 : 18);
+    return cls == 0 ? 1 : cls == 1 ? (esz == 4 ? 4 : 8) : cls == 2 ? (esz == 4 ? 8 : 16) : (esz == 4 ? 9 : 18);
 }
 // tail: classes 0 and 1 keep one register tile per wave free (gcs_geom).
 inline int gcs_jr_class(int esz, uint32_t K, uint32_t KL, uint32_t nw, bool tail = false) {
     const uint32_t extra = K - KL, t = tail ? 1u : 0u;
     if (extra <= ((uint32_t)gcs_jr_of(esz, 0) - t) * nw) return 0;
     if (extra <= ((uint32_t)gcs_jr_of(esz, 1) - t) * nw) return 1;
-    return 2;
+    if (extra <= (uint32_t)gcs_jr_of(esz, 2) * nw) return 2;
+    return 3;
 }
 
 // Which tiles a member of a cooperative group owns.  Strided (bh == 0): member

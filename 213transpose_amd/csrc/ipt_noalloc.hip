@@ -135,7 +135,7 @@ ipt_status ipt_transpose_213_noalloc(void* data, int elem_bytes, int64_t d1, int
         t = resolve_tune(ctx, nullptr);
     }
     if (v == IPT_VARIANT_GCS) {
-        const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, 2, true);
+        const GcsGeom gg = gcs_geom(ctx, s, t.lds_cap, true, 3, true);
         const uint64_t need = (32 + bar_words(gg.g) * (uint64_t)s.d3) * sizeof(uint32_t);
         if (!gg.ok || (gg.g > 1 && need > na->sync.bytes)) {
             v = decomp;

@@ -210,10 +210,12 @@ def test_gcs_any_row_length_and_base(ipt, oracle, esz):
     """The cooperative kernel's 16-byte tiles on rows of any length and on a
     base one element past a 16-byte boundary: partial micro-rows at the right
     and bottom edges, loads that run past the end of the slice, element-wise
-    stores in the last row band.  f64 3751 x 3723 (112 MB) needs the largest
-    register-tile class (18 tiles per wave) to fit one group."""
+    stores in the last row band.  f64 3751 x 3723 (112 MB) and f32 5400 x 5300
+    (114 MB) need the largest register-tile class (18 / 9 tiles per wave) to
+    fit one group."""
     cases = [(1001, 999, 3), (333, 77, 40), (97, 203, 5), (4093, 4099, 1), (35, 33, 300),
-             (1023, 1, 1), (1, 1023, 2), (17, 4099, 2), (4099, 17, 2), (3751, 3723, 1)]
+             (1023, 1, 1), (1, 1023, 2), (17, 4099, 2), (4099, 17, 2), (3751, 3723, 1),
+             (5400, 5300, 1)]
     ran = 0
     for k, (d1, d2, d3) in enumerate(cases):
         if not ipt.feasible(ipt.VARIANT_GCS, esz, d1, d2, d3):
