@@ -348,3 +348,28 @@ def test_batch_barrier_watchdog_reports_and_recovers(ipt, oracle):
     check(oracle, items, host, 1)
     b.close()
     b2.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [11, 12])
+def test_uneven_batches_other_seeds(ipt, oracle, seed):
+    """Uneven batches drawn with other seeds (800 matrices, m and n
+    log-uniform in [2, 3000]): cooperative groups of every size from one
+    member to the whole grid, each member waiting only for the members its
+    stores depend on; every instance exact after an eager run and a graph
+    replay."""
+    shapes = [(n, m, 1, e) for m, n, e in ipt.uneven_shapes(800, 2, 3000, seed=seed)]
+    print(f"[case] uneven batch seed {seed}: "
+          f"{sum(a * b * c * e for a, b, c, e in shapes) / 1e9:.2f} GB", flush=True)
+    buf, items, host = make_instances(oracle, shapes, 20000 + 1000 * seed)
+    b = ipt.Batch(items)
+    print("[batch]", b.info(), flush=True)
+    b.run(graph=False)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, items, host, 1)
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    b.check()
+    check(oracle, items, host, 2)
+    b.close()
