@@ -888,7 +888,8 @@ struct WaveTile {
 // in the last tile column the stores of such a micro-row go element by
 // element up to the row end.  (Only the last tile column has them: I <= J,
 // so a pair whose row band is the last one is a diagonal pair.)
-template <typename T, int TS, int BLOCK>
+// LP, SP: the loads' and stores' cache policies.
+template <typename T, int TS, int BLOCK, int LP = 0, int SP = kStoreDevice>
 __device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, uint64_t t, T* lds, bool dord) {
     constexpr uint32_t TP = TS + 1, PER = 16 / sizeof(T), VPR = TS / PER, VPT = TS * VPR;
     constexpr uint32_t UPT = (VPT + BLOCK - 1) / BLOCK;
@@ -906,8 +907,8 @@ __device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, u
         const uint32_t v = u * BLOCK + threadIdx.x, y = v / VPR, x = (v % VPR) * PER;
         const bool oka = v < VPT && r0 + y < N && c0 + x < N;
         const bool okb = v < VPT && !diag && c0 + y < N && r0 + x < N;
-        ra[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, oka ? ((r0 + y) * N + c0 + x) * E : kOutside, 0, 0);
-        rb[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, okb ? ((c0 + y) * N + r0 + x) * E : kOutside, 0, 0);
+        ra[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, oka ? ((r0 + y) * N + c0 + x) * E : kOutside, 0, LP);
+        rb[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, okb ? ((c0 + y) * N + r0 + x) * E : kOutside, 0, LP);
     }
 #pragma unroll
     for (uint32_t u = 0; u < UPT; u++) {
@@ -940,8 +941,8 @@ __device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, u
             const bool okb = !diag && c0 + y < N && r0 + x < N;
             const bool fa = c0 + x + PER <= N, fb = r0 + x + PER <= N;  // whole micro-row inside
             const uint32_t offa = ((r0 + y) * N + c0 + x) * E, offb = ((c0 + y) * N + r0 + x) * E;
-            __builtin_amdgcn_raw_buffer_store_b128(o.v, rs, oka && fa ? offa : kOutside, 0, kStoreDevice);
-            __builtin_amdgcn_raw_buffer_store_b128(p.v, rs, okb && fb ? offb : kOutside, 0, kStoreDevice);
+            __builtin_amdgcn_raw_buffer_store_b128(o.v, rs, oka && fa ? offa : kOutside, 0, SP);
+            __builtin_amdgcn_raw_buffer_store_b128(p.v, rs, okb && fb ? offb : kOutside, 0, SP);
             if (c0 + TS > N) {  // the last tile column: partial micro-rows (uniform per pair)
 #pragma unroll
                 for (uint32_t q = 0; q < PER; q++) {
@@ -949,14 +950,14 @@ __device__ __forceinline__ void swap_body_rs(T* __restrict__ base, uint32_t N, u
                     const bool wb = okb && !fb && r0 + x + q < N;
                     if constexpr (sizeof(T) == 4) {
                         __builtin_amdgcn_raw_buffer_store_b32((uint32_t)o.e[q], rs, wa ? offa + q * E : kOutside,
-                                                              0, kStoreDevice);
+                                                              0, SP);
                         __builtin_amdgcn_raw_buffer_store_b32((uint32_t)p.e[q], rs, wb ? offb + q * E : kOutside,
-                                                              0, kStoreDevice);
+                                                              0, SP);
                     } else {
                         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (uint64_t)o.e[q]), rs,
-                                                              wa ? offa + q * E : kOutside, 0, kStoreDevice);
+                                                              wa ? offa + q * E : kOutside, 0, SP);
                         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (uint64_t)p.e[q]), rs,
-                                                              wb ? offb + q * E : kOutside, 0, kStoreDevice);
+                                                              wb ? offb + q * E : kOutside, 0, SP);
                     }
                 }
             }
@@ -972,6 +973,23 @@ k_square_swap_rs(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3, 
     for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const uint64_t sl = w / pairs;
         swap_body_rs<T, TS, kSwapBlock>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles, dord);
+        __syncthreads();
+    }
+}
+
+// Large tiles for squares whose row pitch is a power of two of 64 KB or more
+// (512-byte row segments, 1024 threads, one workgroup and its 2 * TS * (TS +
+// 1) elements of LDS per CU), pairs in diagonal order, non-temporal loads and
+// stores.
+constexpr int kSwapBigBlock = 1024;
+template <typename T, int TS>
+__global__ void __launch_bounds__(kSwapBigBlock)
+k_square_swap_big(T* __restrict__ data, uint32_t N, uint64_t pairs, uint64_t d3) {
+    IPT_DYN_LDS(T, tiles);
+    const uint64_t total = pairs * d3;
+    for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const uint64_t sl = w / pairs;
+        swap_body_rs<T, TS, kSwapBigBlock, 2, 2>(data + sl * (uint64_t)N * N, N, w - sl * pairs, tiles, true);
         __syncthreads();
     }
 }
