@@ -157,11 +157,12 @@ static bool slice_lds_fits(const DeviceCtx* ctx, const Shape& s) {
 // every slice fits in one round of one workgroup per CU, slices are spread
 // over all CUs (larger g, less data per member); otherwise members are
 // filled to capacity.  Slices of 2 GiB or more never fit (buffer offsets are
-// 32-bit, and such a slice is far beyond what 256 members hold).  The largest
-// register-tile class is used only when a slice needs it to fit, or when
-// packed groups of it share a round two or more to one of class 1: members
+// 32-bit, and such a slice is far beyond what 256 members hold).  Register
+// tiles of class 2 are used only when a slice needs them to fit, or when
+// packed groups of them share a round two or more to one of class 1: members
 // that large were slower wherever a smaller class also fits (f32 3000 x 3000
-// x 8: 3446 vs 4578 GB/s).
+// x 8: 3446 vs 4578 GB/s); class 3 only when class 2 cannot hold the slice
+// in one group.
 // Packing (several rounds of groups): g members per slice with gmin <= g
 // leave ncu mod g workgroups that start the next round's first group early,
 // and that group's other members then wait at its barrier for them to finish
