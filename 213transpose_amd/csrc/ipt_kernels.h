@@ -1267,7 +1267,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
         if (k < kl) {
             lt[j].transpose();
 #pragma unroll
-            for (uint32_t q = 0; q < W::PER; q++) lds[k * TV + q * 64 + lane] = lt[j].v[q];
+            for (uint32_t q = 0; q < W::PER; q++) lds[k * TV + (W::PER * (lane & 7) + q) * 8 + (lane >> 3)] = lt[j].v[q];
         }
     }
 #pragma unroll
@@ -1295,18 +1295,68 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     }
     __syncthreads();
     released();
+    // Staged tiles: LDS unit k * TV + R_o * 8 + c holds 16-byte chunk c of
+    // output row R_o of tile k.  In the batch kernel, when the member's staged
+    // tiles lie in full bands, each store writes one output row across the
+    // band's eight tiles of one tile column (1 KB runs, lane r * 8 + c: tile
+    // row r, chunk c) instead of each wave writing its own tiles' 128-byte
+    // segments: uneven batch +0.3-0.6 % (4268-4279 -> 4280-4298 GB/s over
+    // two A/B runs of three).
+    bool runs = false;
+    if constexpr (AL) runs = ord.bh == NW && kl > 0 && m * K + kl <= ord.full * ord.dband.d;
+    // Positions P0 .. P1 - 1 of the member's tiles, staged in LDS at unit
+    // (p - P0) * TV, stored as runs.
+    auto store_runs = [&](uint32_t P0, uint32_t P1) {
+        constexpr uint32_t E_ = sizeof(T);
+        const uint32_t G0 = P0 / NW * NW;
+        const uint32_t ng = (P1 + NW - 1) / NW - P0 / NW;
+        const uint32_t r = lane >> 3, c = lane & 7;
+        for (uint32_t run = wave; run < ng * W::TS; run += NW) {
+            const uint32_t gi = run / W::TS, ro = run - gi * W::TS;
+            const uint32_t gp = G0 + NW * gi;  // first position of the tile column
+            const uint32_t p = gp + r;
+            const bool valid = p >= P0 && p < P1;
+            const u32x4 x = lds[(valid ? p - P0 : 0u) * TV + ro * 8 + c];
+            const uint32_t b = ord.dband.div(gp), tj = (gp - b * ord.dband.d) / NW;
+            const uint32_t orow = tj * W::TS + ro, ocol = (b * NW + r) * W::TS + W::PER * c;
+            const bool ok = valid && orow < C && ocol < R;
+            const bool full = ocol + W::PER <= R;
+            __builtin_amdgcn_raw_buffer_store_b128(x, rs, ok && full ? (orow * R + ocol) * E_ : kOutside, 0, SP);
+            if ((b + 1) * NW * W::TS > R) {  // the band holds the last, partial tile row
+                union { u32x4 v; T e[W::PER]; } y;
+                y.v = x;
 #pragma unroll
-    for (int j = 0; j < JL; j++) {
-        const uint32_t k = wave + j * NW;
-        if (k < kl) {
-            W x;
+                for (uint32_t e = 0; e < W::PER; e++) {
+                    const uint32_t off = ok && !full && ocol + e < R ? (orow * R + ocol + e) * E_ : kOutside;
+                    if constexpr (sizeof(T) == 4)
+                        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)y.e[e], rs, off, 0, SP);
+                    else
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (uint64_t)y.e[e]), rs,
+                                                              off, 0, SP);
+                }
+            }
+        }
+    };
+    if (runs) {
+        store_runs(m * K, m * K + kl);
+    } else {
 #pragma unroll
-            for (uint32_t q = 0; q < W::PER; q++) x.v[q] = lds[k * TV + q * 64 + lane];
-            uint32_t r0, c0;
-            origin(k, &r0, &c0);
-            x.template store_t<SP>(rs, r0, c0, R, C, lane, true);
+        for (int j = 0; j < JL; j++) {
+            const uint32_t k = wave + j * NW;
+            if (k < kl) {
+                W x;
+#pragma unroll
+                for (uint32_t q = 0; q < W::PER; q++)
+                    x.v[q] = lds[k * TV + (W::PER * (lane & 7) + q) * 8 + (lane >> 3)];
+                uint32_t r0, c0;
+                origin(k, &r0, &c0);
+                x.template store_t<SP>(rs, r0, c0, R, C, lane, true);
+            }
         }
     }
+    // Register tiles: each wave its own (restaging them through LDS to store
+    // them as runs too cost two more workgroup barriers: uneven batch 4268
+    // -> 3948 GB/s).
 #pragma unroll
     for (int j = 0; j < JR; j++) {
         const uint32_t k = rk(j);
