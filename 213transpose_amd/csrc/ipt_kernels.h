@@ -1162,6 +1162,17 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     constexpr uint32_t NW = BLOCK / 64;
     constexpr int JL = gcs_wave_lds_tiles(sizeof(T));
     constexpr uint32_t TV = 64 * W::PER;  // 16-byte vectors per staged tile
+    // LDS unit of 16-byte chunk c of output row ro of staged tile k: rows 8
+    // units (128 bytes) apart, the chunk XORed with the row's PER-row group
+    // and the row's low bit flipped by the parity of c + k.  A transposing
+    // write (lanes 8b + a: row PER * a + q, chunk b) and a run read (lanes
+    // 8r + c: tile r, chunk c) then put any 8 consecutive lanes on distinct
+    // 16-byte slots of 128 bytes and any 16 on distinct slots of 256 bytes;
+    // unswizzled, the eight lanes of a write shared one slot (k_batch: 2/3 of
+    // its LDS cycles were bank conflicts).
+    auto unit = [](uint32_t k, uint32_t ro, uint32_t c) {
+        return k * TV + (ro ^ ((c ^ k) & 1u)) * 8 + (c ^ ((ro / W::PER) & 7u));
+    };
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nk = ord.bh == 0 ? (m < ntiles ? min(K, (ntiles - m + g - 1) / g) : 0)
@@ -1267,7 +1278,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
         if (k < kl) {
             lt[j].transpose();
 #pragma unroll
-            for (uint32_t q = 0; q < W::PER; q++) lds[k * TV + (W::PER * (lane & 7) + q) * 8 + (lane >> 3)] = lt[j].v[q];
+            for (uint32_t q = 0; q < W::PER; q++) lds[unit(k, W::PER * (lane & 7) + q, lane >> 3)] = lt[j].v[q];
         }
     }
 #pragma unroll
@@ -1295,8 +1306,8 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     }
     __syncthreads();
     released();
-    // Staged tiles: LDS unit k * TV + R_o * 8 + c holds 16-byte chunk c of
-    // output row R_o of tile k.  In the batch kernel, when the member's staged
+    // Staged tiles: LDS unit(k, R_o, c) holds 16-byte chunk c of output row
+    // R_o of tile k.  In the batch kernel, when the member's staged
     // tiles lie in full bands, each store writes one output row across the
     // band's eight tiles of one tile column (1 KB runs, lane r * 8 + c: tile
     // row r, chunk c) instead of each wave writing its own tiles' 128-byte
@@ -1304,8 +1315,8 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
     // two A/B runs of three).
     bool runs = false;
     if constexpr (AL) runs = ord.bh == NW && kl > 0 && m * K + kl <= ord.full * ord.dband.d;
-    // Positions P0 .. P1 - 1 of the member's tiles, staged in LDS at unit
-    // (p - P0) * TV, stored as runs.
+    // Positions P0 .. P1 - 1 of the member's tiles, staged in LDS as tiles
+    // p - P0, stored as runs.
     auto store_runs = [&](uint32_t P0, uint32_t P1) {
         constexpr uint32_t E_ = sizeof(T);
         const uint32_t G0 = P0 / NW * NW;
@@ -1316,7 +1327,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
             const uint32_t gp = G0 + NW * gi;  // first position of the tile column
             const uint32_t p = gp + r;
             const bool valid = p >= P0 && p < P1;
-            const u32x4 x = lds[(valid ? p - P0 : 0u) * TV + ro * 8 + c];
+            const u32x4 x = lds[unit(valid ? p - P0 : 0u, ro, c)];
             const uint32_t b = ord.dband.div(gp), tj = (gp - b * ord.dband.d) / NW;
             const uint32_t orow = tj * W::TS + ro, ocol = (b * NW + r) * W::TS + W::PER * c;
             const bool ok = valid && orow < C && ocol < R;
@@ -1347,7 +1358,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
                 W x;
 #pragma unroll
                 for (uint32_t q = 0; q < W::PER; q++)
-                    x.v[q] = lds[k * TV + (W::PER * (lane & 7) + q) * 8 + (lane >> 3)];
+                    x.v[q] = lds[unit(k, W::PER * (lane & 7) + q, lane >> 3)];
                 uint32_t r0, c0;
                 origin(k, &r0, &c0);
                 x.template store_t<SP>(rs, r0, c0, R, C, lane, true);
