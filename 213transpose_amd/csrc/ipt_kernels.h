@@ -1321,29 +1321,39 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
         constexpr uint32_t E_ = sizeof(T);
         const uint32_t G0 = P0 / NW * NW;
         const uint32_t ng = (P1 + NW - 1) / NW - P0 / NW;
+        static_assert(W::TS % NW == 0, "whole rows per wave");
         const uint32_t r = lane >> 3, c = lane & 7;
-        for (uint32_t run = wave; run < ng * W::TS; run += NW) {
-            const uint32_t gi = run / W::TS, ro = run - gi * W::TS;
+        // wave w stores rows w, w + NW, .. of each tile column in turn
+        for (uint32_t gi = 0; gi < ng; gi++) {
             const uint32_t gp = G0 + NW * gi;  // first position of the tile column
             const uint32_t p = gp + r;
             const bool valid = p >= P0 && p < P1;
-            const u32x4 x = lds[unit(valid ? p - P0 : 0u, ro, c)];
+            const uint32_t kk = valid ? p - P0 : 0u;
             const uint32_t b = ord.dband.div(gp), tj = (gp - b * ord.dband.d) / NW;
-            const uint32_t orow = tj * W::TS + ro, ocol = (b * NW + r) * W::TS + W::PER * c;
-            const bool ok = valid && orow < C && ocol < R;
+            const uint32_t ocol = (b * NW + r) * W::TS + W::PER * c;
+            const bool vcol = valid && ocol < R;
             const bool full = ocol + W::PER <= R;
-            __builtin_amdgcn_raw_buffer_store_b128(x, rs, ok && full ? (orow * R + ocol) * E_ : kOutside, 0, SP);
-            if ((b + 1) * NW * W::TS > R) {  // the band holds the last, partial tile row
-                union { u32x4 v; T e[W::PER]; } y;
-                y.v = x;
+            const bool part = (b + 1) * NW * W::TS > R;  // the band holds the last, partial tile row
 #pragma unroll
-                for (uint32_t e = 0; e < W::PER; e++) {
-                    const uint32_t off = ok && !full && ocol + e < R ? (orow * R + ocol + e) * E_ : kOutside;
-                    if constexpr (sizeof(T) == 4)
-                        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)y.e[e], rs, off, 0, SP);
-                    else
-                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (uint64_t)y.e[e]), rs,
-                                                              off, 0, SP);
+            for (uint32_t i = 0; i < W::TS / NW; i++) {
+                const uint32_t ro = wave + i * NW;
+                const u32x4 x = lds[unit(kk, ro, c)];
+                const uint32_t orow = tj * W::TS + ro;
+                const bool ok = vcol && orow < C;
+                const uint32_t o = (orow * R + ocol) * E_;
+                __builtin_amdgcn_raw_buffer_store_b128(x, rs, ok && full ? o : kOutside, 0, SP);
+                if (part) {
+                    union { u32x4 v; T e[W::PER]; } y;
+                    y.v = x;
+#pragma unroll
+                    for (uint32_t e = 0; e < W::PER; e++) {
+                        const uint32_t off = ok && !full && ocol + e < R ? o + e * E_ : kOutside;
+                        if constexpr (sizeof(T) == 4)
+                            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)y.e[e], rs, off, 0, SP);
+                        else
+                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (uint64_t)y.e[e]),
+                                                                  rs, off, 0, SP);
+                    }
                 }
             }
         }
