@@ -857,7 +857,7 @@ struct WaveTile {
         for (uint32_t q = 0; q < PER; q++)
             __builtin_amdgcn_raw_buffer_store_b128(
                 v[q], rs, ok && full && orow + q < cols ? o + q * rows * E : kOutside, 0, SP);
-        if (r0 + TS > rows) {  // last row band (uniform across the wave)
+        if (r0 + TS > rows && __any(ok && !full)) {  // last row band, some lane at a partial row end
 #pragma unroll
             for (uint32_t q = 0; q < PER; q++) {
                 union { u32x4 v; T e[PER]; } x;
@@ -1342,7 +1342,7 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
                 const bool ok = vcol && orow < C;
                 const uint32_t o = (orow * R + ocol) * E_;
                 __builtin_amdgcn_raw_buffer_store_b128(x, rs, ok && full ? o : kOutside, 0, SP);
-                if (part) {
+                if (part && __any(ok && !full)) {  // some lane holds a row's partial end
                     union { u32x4 v; T e[W::PER]; } y;
                     y.v = x;
 #pragma unroll
