@@ -760,6 +760,19 @@ __device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
 // columns >= rows.  Output rows >= cols are not stored; an output micro-row
 // that runs past `rows` (only in tiles of the last row band) is stored
 // element by element, up to `rows`.
+// The first n (1 .. PER - 1) elements of the 16-byte vector x to byte
+// offset o (lanes with live false store nothing): 4-byte elements as one
+// 8-byte store of elements 0-1 when n >= 2 and one 4-byte store of element
+// n - 1 when n is odd, 8-byte elements as one 8-byte store.
+template <typename T, int SP>
+__device__ __forceinline__ void store_head(u32x4 x, Rsrc rs, uint32_t o, uint32_t n, bool live) {
+    __builtin_amdgcn_raw_buffer_store_b64(u32x2{x[0], x[1]}, rs,
+                                          live && (sizeof(T) == 8 || n >= 2) ? o : kOutside, 0, SP);
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(n == 1 ? x[0] : x[2], rs, live && (n & 1) ? o + (n - 1) * 4 : kOutside,
+                                              0, SP);
+}
+
 template <typename T>
 struct WaveTile {
     static constexpr uint32_t PER = 16 / sizeof(T);
@@ -859,20 +872,8 @@ struct WaveTile {
                 v[q], rs, ok && full && orow + q < cols ? o + q * rows * E : kOutside, 0, SP);
         if (r0 + TS > rows && __any(ok && !full)) {  // last row band, some lane at a partial row end
 #pragma unroll
-            for (uint32_t q = 0; q < PER; q++) {
-                union { u32x4 v; T e[PER]; } x;
-                x.v = v[q];
-#pragma unroll
-                for (uint32_t e = 0; e < PER; e++) {
-                    const bool w = ok && !full && orow + q < cols && ocol + e < rows;
-                    const uint32_t off = w ? o + (q * rows + e) * E : kOutside;
-                    if constexpr (sizeof(T) == 4)
-                        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)x.e[e], rs, off, 0, SP);
-                    else
-                        __builtin_amdgcn_raw_buffer_store_b64(
-                            __builtin_bit_cast(u32x2, (uint64_t)x.e[e]), rs, off, 0, SP);
-                }
-            }
+            for (uint32_t q = 0; q < PER; q++)
+                store_head<T, SP>(v[q], rs, o + q * rows * E, rows - ocol, ok && !full && orow + q < cols);
         }
     }
 };
@@ -1342,19 +1343,8 @@ __device__ __forceinline__ void gcs_body(T* __restrict__ base, uint32_t R, uint3
                 const bool ok = vcol && orow < C;
                 const uint32_t o = (orow * R + ocol) * E_;
                 __builtin_amdgcn_raw_buffer_store_b128(x, rs, ok && full ? o : kOutside, 0, SP);
-                if (part && __any(ok && !full)) {  // some lane holds a row's partial end
-                    union { u32x4 v; T e[W::PER]; } y;
-                    y.v = x;
-#pragma unroll
-                    for (uint32_t e = 0; e < W::PER; e++) {
-                        const uint32_t off = ok && !full && ocol + e < R ? o + e * E_ : kOutside;
-                        if constexpr (sizeof(T) == 4)
-                            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)y.e[e], rs, off, 0, SP);
-                        else
-                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (uint64_t)y.e[e]),
-                                                                  rs, off, 0, SP);
-                    }
-                }
+                if (part && __any(ok && !full))  // some lane holds a row's partial end
+                    store_head<T, SP>(x, rs, o, R - ocol, ok && !full);
             }
         }
     };
